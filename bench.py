@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""ES-PIC inner-loop benchmark on one MI355X.
+
+Workload (default): flow past a sphere on a 257x257x513 grid over [-0.1,0.1]^2 x [0,0.4] m,
+sphere r=0.05 m at (0,0,0.15) at -100 V, inlet k=0 at 0 V; 1e9 O+ macroparticles loaded
+uniformly with a 7 km/s drift along z plus a 1000 K isotropic thermal spread (Philox,
+seed 1), a warm beam injected at z=0 every step (n0=1e10 m^-3); Boltzmann electrons
+(n0=1e10, Te=1.5 eV, phi0=0) solved by Newton + Jacobi-PCG (tol 1e-4, NR tol 1e-3), dt=1e-7 s.
+
+One step = inject -> advance (fused push + CIC gather + CIC deposit + rebin) ->
+computeNumberDensity -> computeChargeDensity -> solve -> computeEF, the call sequence of
+ch3/ver2/Main.cpp:50-70.  The headline value is particle-steps/s of push+gather+deposit
+(HIP events around those launches on the world's stream), mixed precision; the same state
+is then run with FP64 particles.  Poisson ms/step is reported beside it.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "plasma-simulations-by-example_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+QE = 1.602176565e-19
+AMU = 1.660538921e-27
+K_B = 1.380648e-23
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_PER_PARTICLE_STEP = 60  # read 32 (cell, 3 offsets, 3 v, w) + write 28 (cell, offsets, v)
+BYTES_PER_NODE_E = 12         # FP32 E read per node
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--particles", type=float, default=1e9)
+    ap.add_argument("--grid", type=str, default="257,257,513")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference timing")
+    ap.add_argument("--cpu-particles", type=float, default=1e7)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--T-ion", type=float, default=1000.0)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Case:
+    x0 = (-0.1, -0.1, 0.0)
+    xm = (0.1, 0.1, 0.4)
+    sphere_c = (0.0, 0.0, 0.15)
+    sphere_r = 0.05
+    phi_s = -100.0
+    n0 = 1e10
+    v_drift = 7000.0
+    Te = 1.5
+    dt = 1e-7
+
+
+def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=False):
+    """Builds the sphere case and times `steps` steps.  Returns a dict of results."""
+    ni, nj, nk = grid
+    w = P.World(ni, nj, nk)
+    w.set_extents(Case.x0, Case.xm)
+    w.add_sphere(Case.sphere_c, Case.sphere_r, Case.phi_s)
+    w.add_inlet()
+    vol = np.prod(np.subtract(Case.xm, Case.x0))
+    mpw0 = Case.n0 * vol / nparticles
+    ions = P.Species(w, 16 * AMU, QE, mpw0, precision, capacity=int(nparticles * 1.02), name="O+")
+    t0 = time.time()
+    ions.load_box_device(Case.x0, Case.xm, Case.n0, int(nparticles), Case.v_drift, T_ion, seed=1)
+    w.sync()
+    t_load = time.time() - t0
+    solver = P.PotentialSolver(w, P.PCG, 10000, 1e-4)
+    solver.set_reference_values(0.0, Case.Te, Case.n0)
+    solver.solve_qn()        # the ch3 constructor's initial guess (buildMatrix -> solveQN)
+    solver.compute_ef()
+
+    counter = [0]
+
+    def step():
+        ions.inject_beam(Case.v_drift, Case.n0, T_ion, Case.dt, seed=1, counter=counter[0])
+        counter[0] += 1
+        ions.advance(P.ABSORB, Case.dt)
+        ions.compute_number_density()
+        w.compute_charge_density([ions])
+        ok, res, its = solver.solve()
+        solver.compute_ef()
+        return ok, its
+
+    log(f"[bench] {'mixed' if precision == P.MIXED else 'fp64'}: loaded {nparticles:.3g} "
+        f"particles in {t_load:.1f}s; warmup {warmup}")
+    for i in range(warmup):
+        ok, its = step()
+        log(f"[bench]   warmup {i}: pcg its {its} converged {ok} np {ions.np():.4g}")
+    w.set_timing(True)
+    w.timers(reset=True)
+    np_sum = 0
+    pcg = []
+    w.sync()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        n_before = ions.np()
+        ok, its = step()
+        np_sum += n_before
+        pcg.append(its)
+    w.sync()
+    wall = time.perf_counter() - t0
+    tm = w.timers(reset=True)
+    w.set_timing(False)
+    push_ms, push_calls = tm["push"]
+    dep_ms, _ = tm["deposit"]
+    out = dict(
+        np_final=ions.np(), particle_steps=np_sum, wall_s=wall, ms_per_step=1e3 * wall / steps,
+        push_ms=push_ms, deposit_ms=dep_ms, push_calls=push_calls,
+        inject_ms=tm["inject"][0], solve_ms=tm["solve"][0], ef_ms=tm["ef"][0], rho_ms=tm["rho"][0],
+        pcg_iters=pcg, load_s=t_load, nn=ni * nj * nk,
+    )
+    out["particle_steps_per_s"] = np_sum / ((push_ms + dep_ms) * 1e-3)
+    out["push_avg_ms"] = push_ms / max(push_calls, 1)
+    if want_fields:
+        out["ef"] = w.field(P.EF)
+        out["phi"] = w.field(P.PHI)
+    ions.close()
+    solver.close()
+    w.close()
+    return out
+
+
+def cpu_baseline(args, grid, ef, T_ion):
+    """Times the reference's own Species::advance + computeNumberDensity (oracle/_ref,
+    built from the reference sources) on a bounded sample of the same workload: the same
+    grid, sphere and E field, cpu-particles particles of the same distribution."""
+    from oracle import ref as R
+    if not (R.available("ch3") and R.available("mt")):
+        return None
+    ni, nj, nk = grid
+    n = int(args.cpu_particles)
+    rng = np.random.default_rng(7)
+    lo, hi = np.array(Case.x0), np.array(Case.xm)
+    pos = lo + rng.random((n, 3)) * (hi - lo)
+    c = np.array(Case.sphere_c)
+    pos = pos[np.sum((pos - c) ** 2, axis=1) > Case.sphere_r ** 2]
+    n = len(pos)
+    vth = np.sqrt(2 * K_B * T_ion / (16 * AMU))
+    vel = rng.normal(0, vth / np.sqrt(2), (n, 3))
+    vel[:, 2] += Case.v_drift
+    vol = np.prod(hi - lo)
+    mpw = np.full(n, Case.n0 * vol / args.particles)
+    res = {}
+    for kind, ch, threads in (("serial", "ch3", 1), ("mt", "mt", args.cpu_threads)):
+        sim = R.RefSim(ch, ni, nj, nk, Case.x0, Case.xm, Case.dt, 100)
+        sim.add_sphere(Case.sphere_c, Case.sphere_r, Case.phi_s)
+        sim.add_inlet()
+        sp = sim.add_species("O+", 16 * AMU, QE, float(mpw[0]))
+        if ch == "mt":
+            sim.set_num_threads(threads)
+        sim.set_field(R.EF, ef)
+        sim.set_particles(sp, pos, vel, mpw)
+        t, ps = sim.time_particle_steps(args.cpu_steps)
+        res[kind] = dict(value=ps / t, seconds=t, particle_steps=ps, threads=threads)
+        sim.close()
+        log(f"[bench] cpu {kind}: {ps / t:.3e} particle-steps/s ({t:.1f}s, {threads} threads)")
+    return res, n
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus "
+              f"{args.gpus}. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import pic_amd as P
+    if P.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    grid = tuple(int(x) for x in args.grid.split(","))
+    mixed = run_gpu(P, grid, args.particles, P.MIXED, args.steps, args.warmup, args.T_ion,
+                    want_fields=not args.no_cpu)
+    log(f"[bench] mixed: {mixed['particle_steps_per_s']:.4e} particle-steps/s, "
+        f"push {mixed['push_avg_ms']:.3f} ms/launch, {mixed['ms_per_step']:.1f} ms/step")
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_gpu(P, grid, args.particles, P.FP64, args.steps, args.warmup, args.T_ion)
+        log(f"[bench] fp64: {fp64['particle_steps_per_s']:.4e} particle-steps/s, "
+            f"push {fp64['push_avg_ms']:.3f} ms/launch")
+
+    # roofline of the dominant kernel (the fused push): algorithmic bytes per launch over
+    # its average HIP-event duration
+    steps = args.steps
+    np_avg = mixed["particle_steps"] / steps
+    bytes_launch = np_avg * BYTES_PER_PARTICLE_STEP + mixed["nn"] * BYTES_PER_NODE_E
+    achieved = bytes_launch / (mixed["push_avg_ms"] * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "push_mixed_sphere257_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "kernel": "k_push<PIC_MIXED>", "bytes_per_launch": bytes_launch}
+
+    cpu = None
+    if not args.no_cpu:
+        r = cpu_baseline(args, grid, mixed["ef"], args.T_ion)
+        if r is not None:
+            res, n = r
+            cpu = {"value": res["serial"]["value"], "unit": "particle-steps/s", "cores": 1,
+                   "kind": "reference",
+                   "sample": f"reference ch3/ver2 Species::advance+computeNumberDensity, "
+                             f"{n} particles x {args.cpu_steps} steps on the same "
+                             f"{grid[0]}x{grid[1]}x{grid[2]} grid, sphere and E field",
+                   "threaded": {"value": res["mt"]["value"], "cores": res["mt"]["threads"],
+                                "kind": "reference", "code": "ch9/MT"}}
+
+    line = {
+        "metric": "particle-steps/s (push+gather+deposit) + Poisson ms/step, mixed vs FP64",
+        "value": mixed["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"sphere {grid[0]}x{grid[1]}x{grid[2]}, {args.particles:.3g} O+ "
+                        f"mixed-precision particles, Boltzmann-electron Newton-PCG",
+            "particles": args.particles, "grid": list(grid), "dt": Case.dt,
+            "solver": "NR + Jacobi-PCG, FP64, tol 1e-4, NR tol 1e-3",
+            "particles_final": mixed["np_final"],
+        },
+        "poisson_ms_per_step": mixed["solve_ms"] / steps,
+        "pcg_iters_per_step": float(np.mean(mixed["pcg_iters"])),
+        "phase_ms_per_step": {k: mixed[k + "_ms"] / steps for k in
+                              ("push", "deposit", "inject", "solve", "ef", "rho")},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if fp64 is not None:
+        line["fp64"] = {"value": fp64["particle_steps_per_s"], "unit": "particle-steps/s",
+                        "ms_per_step": fp64["ms_per_step"],
+                        "poisson_ms_per_step": fp64["solve_ms"] / steps,
+                        "push_avg_ms": fp64["push_avg_ms"],
+                        "mixed_speedup": mixed["particle_steps_per_s"] / fp64["particle_steps_per_s"]}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

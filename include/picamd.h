@@ -1,0 +1,187 @@
+/*
+ * picamd.h -- C ABI of the MI355X (gfx950) ES-PIC inner loop.
+ *
+ * The reference has no FFI: its programs compile World / Species / PotentialSolver
+ * straight into Main.cpp.  This ABI is the boundary those classes call through, one
+ * entry point per reference operation on the hot path (the reference function each one
+ * replaces is cited beside it).  The C++ drop-in classes in
+ * plasma-simulations-by-example_amd/host/ and the Python binding pic_amd.py both bind
+ * these symbols; nothing else is exported.
+ *
+ * Conventions
+ *  - Plain C types only; device memory is owned by the library behind opaque handles.
+ *    Host arrays are passed as pointers + sizes and copied synchronously.
+ *  - Node arrays use the reference's flat index U = k*ni*nj + j*ni + i
+ *    (ch3/ver2/Field.h:161); 3-vectors are interleaved (x,y,z per node / particle).
+ *  - Every call returns an int status: PIC_OK (0), PIC_NOT_CONVERGED (1, solver only,
+ *    mirrors `bool solve()` returning false), or a negative error.  Nothing throws
+ *    across the ABI; pic_last_error() holds a message for the calling thread.
+ *  - All work of one world is ordered on that world's own HIP stream (default
+ *    priority).  Calls that return host data synchronise that stream first.
+ */
+#ifndef PICAMD_H
+#define PICAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PICAMD_ABI_VERSION 1
+
+/* status codes */
+#define PIC_OK 0
+#define PIC_NOT_CONVERGED 1
+#define PIC_ERR_ARG (-1)
+#define PIC_ERR_HIP (-2)
+#define PIC_ERR_CAPACITY (-3)
+#define PIC_ERR_STATE (-4)
+#define PIC_ERR_NODEVICE (-5)
+
+/* particle storage / arithmetic of a species */
+#define PIC_MIXED 0 /* int32 packed cell + FP32 in-cell offset, FP32 velocity and weight */
+#define PIC_FP64 1  /* FP64 physical position, velocity and weight (the reference's form) */
+
+/* wall treatment in advance() */
+#define PIC_BC_REFLECT 0 /* specular walls, ch2/Species.cpp:32-36 */
+#define PIC_BC_ABSORB 1  /* absorbing walls and sphere, ch3/ver2/Species.cpp:28-32 */
+
+/* solver types: GS, PCG, QN as ch3/ver2/PotentialSolver.h:40; SOR_LINEAR is the ch2 solver */
+#define PIC_SOLVER_GS 0
+#define PIC_SOLVER_PCG 1
+#define PIC_SOLVER_QN 2
+#define PIC_SOLVER_SOR_LINEAR 3
+
+/* world node fields for pic_world_get_field / pic_world_set_field */
+#define PIC_FIELD_PHI 0       /* World::phi */
+#define PIC_FIELD_RHO 1       /* World::rho */
+#define PIC_FIELD_NODE_VOL 2  /* World::node_vol */
+#define PIC_FIELD_EF 3        /* World::ef (3 doubles per node) */
+#define PIC_FIELD_OBJECT_ID 4 /* World::object_id (as double) */
+
+/* phase timers (pic_world_get_timers) */
+#define PIC_T_PUSH 0    /* fused push + gather + deposit + rebin */
+#define PIC_T_DEPOSIT 1 /* density finish (window reduction, /node_vol) */
+#define PIC_T_SORT 2    /* full counting sort */
+#define PIC_T_SOLVE 3   /* Poisson solve */
+#define PIC_T_EF 4      /* computeEF */
+#define PIC_T_RHO 5     /* computeChargeDensity */
+#define PIC_T_INJECT 6  /* device injection */
+#define PIC_T_COUNT 7
+
+typedef struct pic_world pic_world;
+typedef struct pic_species pic_species;
+typedef struct pic_solver pic_solver;
+
+/* ---- library ---------------------------------------------------------------------- */
+int pic_abi_version(void);
+const char *pic_last_error(void);
+/* number of visible HIP devices (0 without a GPU; no other call needs one to be absent) */
+int pic_device_count(int *n);
+
+/* ---- World (ch3/ver2/World.h:38-147) ---------------------------------------------- */
+/* World::World(ni,nj,nk) (ch3/ver2/World.cpp:14-20): allocates the node fields on the
+ * current device and creates the world's stream. */
+int pic_world_create(int ni, int nj, int nk, pic_world **out);
+int pic_world_destroy(pic_world *w);
+/* World::setExtents (ch3/ver2/World.cpp:23-37): dh = (xm-x0)/(nn-1); node volumes. */
+int pic_world_set_extents(pic_world *w, const double x0[3], const double xm[3]);
+/* World::addSphere (ch3/ver2/World.cpp:88-106): object_id=1, phi=phi_s on nodes with
+ * |x-c|^2 <= r^2; the sphere also absorbs particles in advance(PIC_BC_ABSORB). */
+int pic_world_add_sphere(pic_world *w, const double c[3], double r, double phi_s);
+/* World::addInlet (ch3/ver2/World.cpp:109-116): k=0 plane object_id=2, phi=0. */
+int pic_world_add_inlet(pic_world *w);
+/* Host copy of a node field (the updateHostPhi() boundary of ch9/CUDA/PotentialSolver.cu:
+ * 323-329).  out holds nn (or 3*nn for PIC_FIELD_EF) doubles. */
+int pic_world_get_field(pic_world *w, int which, double *out);
+/* Host write of PIC_FIELD_PHI, PIC_FIELD_RHO or PIC_FIELD_EF. */
+int pic_world_set_field(pic_world *w, int which, const double *in);
+/* World::computeChargeDensity (ch3/ver2/World.cpp:47-55): rho = sum q_s*den_s over
+ * charged species (den_s must be current: call pic_species_compute_number_density). */
+int pic_world_compute_charge_density(pic_world *w, pic_species *const *sp, int nsp);
+/* World::getPE (ch3/ver2/World.cpp:73-85): 0.5*eps0*sum |E|^2 * node_vol (FP64). */
+int pic_world_get_pe(pic_world *w, double *pe);
+/* Waits for all queued work of the world. */
+int pic_world_sync(pic_world *w);
+/* Raw hipStream_t of the world (for callers that time or overlap work on it). */
+int pic_world_stream(pic_world *w, void **stream);
+/* Phase timers (HIP events around each launch; off by default).  ms[PIC_T_COUNT] receives
+ * accumulated milliseconds, calls[PIC_T_COUNT] launch counts; reset clears them. */
+int pic_world_set_timing(pic_world *w, int enable);
+int pic_world_get_timers(pic_world *w, double *ms, long *calls, int reset);
+
+/* ---- Species (ch3/ver2/Species.h:11-72) ------------------------------------------- */
+/* Species(name, mass, charge, mpw0, world).  precision: PIC_MIXED or PIC_FP64.
+ * capacity: particles the species may hold (grown on demand by the host calls that add
+ * particles).  track_ids: carry a uint32 id per particle (for per-particle parity). */
+int pic_species_create(pic_world *w, double mass, double charge, double mpw0, int precision,
+                       long capacity, int track_ids, pic_species **out);
+int pic_species_destroy(pic_species *s);
+/* Species::addParticle (ch3/ver2/Species.cpp:65-81) for n particles: rejects positions
+ * outside [x0,xm), rewinds v by (q/m)*E(x)*dt/2 with the current E, appends.
+ * *n_added (nullable) receives the number appended.  ids are assigned in call order. */
+int pic_species_add_particles(pic_species *s, long n, const double *pos, const double *vel,
+                              const double *mpw, double dt, long *n_added);
+/* Raw replacement of all particles (no rewind, no bounds rejection); ids nullable. */
+int pic_species_set_particles(pic_species *s, long n, const double *pos, const double *vel,
+                              const double *mpw, const uint32_t *ids);
+/* Host copy of all particles in device order (pos/vel: 3*np doubles). ids nullable. */
+int pic_species_get_particles(pic_species *s, double *pos, double *vel, double *mpw,
+                              uint32_t *ids);
+/* Species::getNp (ch3/ver2/Species.h:29) */
+int pic_species_get_np(pic_species *s, long *np);
+/* Species::advance: fused leapfrog push + CIC gather (+ CIC deposit of the new positions,
+ * consumed by computeNumberDensity) with reflecting (ch2/Species.cpp:7-38) or absorbing
+ * walls/sphere and compaction (ch3/ver2/Species.cpp:7-48). */
+int pic_species_advance(pic_species *s, int bc, double dt);
+/* Species::computeNumberDensity (ch3/ver2/Species.cpp:51-62): den = scatter(mpw)/node_vol.
+ * Reuses the deposit fused into advance when nothing changed since; deposits afresh
+ * otherwise. */
+int pic_species_compute_number_density(pic_species *s);
+/* Host copy of Species::den / den_ave (nn doubles). */
+int pic_species_get_den(pic_species *s, double *out);
+int pic_species_get_den_ave(pic_species *s, double *out);
+/* Species::updateAverages -> Field_::updateAverage (ch3/ver2/Field.h:214-220) */
+int pic_species_update_averages(pic_species *s);
+/* Species::getRealCount / getMomentum / getKE (ch3/ver2/Species.cpp:84-108), FP64
+ * reductions.  out[5] = real count, px, py, pz, KE. */
+int pic_species_diag(pic_species *s, double out[5]);
+/* Counting sort of all particles into tile bins (tile-major cell order). */
+int pic_species_sort(pic_species *s);
+/* ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27) when T == 0, WarmBeamSource::sample
+ * (ch4/Source.cpp:31-56) when T > 0 [K], on the device: num = (int)(den*v*A*dt/mpw0 + U)
+ * particles uniform on the k=0 face, v = v_drift z (+ isotropic thermal speed), then the
+ * addParticle rewind.  Random numbers: Philox4x32-10 keyed by (seed, counter), so a run
+ * is reproducible; *n_added (nullable) receives the count. */
+int pic_species_inject_beam(pic_species *s, double v_drift, double den, double T, double dt,
+                            uint64_t seed, uint64_t counter, long *n_added);
+/* Device bulk load of num_mp particles uniform in [x1,x2) with zero velocity and weight
+ * num_den*V/num_mp (Species::loadParticlesBox, ch2/Species.cpp:74-97), Philox random
+ * numbers; for the 1e8-1e9 particle workloads that a host load cannot feed. Optional
+ * drift/thermal velocity as inject_beam. */
+int pic_species_load_box_device(pic_species *s, const double x1[3], const double x2[3],
+                                double num_den, long num_mp, double v_drift, double T,
+                                uint64_t seed);
+
+/* ---- PotentialSolver (ch3/ver2/PotentialSolver.h:40-119; ch2/PotentialSolver.h) ---- */
+/* PotentialSolver(world, type, max_it, tol).  Unlike the reference constructor this does
+ * not run solveQN(); the drop-in class calls pic_solver_solve_qn to reproduce it. */
+int pic_solver_create(pic_world *w, int type, int max_it, double tol, pic_solver **out);
+int pic_solver_destroy(pic_solver *s);
+/* setReferenceValues(phi0, Te0, n0) -- note Te0 second, as the reference. */
+int pic_solver_set_reference_values(pic_solver *s, double phi0, double Te0, double n0);
+/* solve(): PIC_OK when converged, PIC_NOT_CONVERGED otherwise.  residual (nullable) gets
+ * the last convergence norm, iters (nullable) the iterations (PCG: total inner PCG
+ * iterations over the Newton steps). */
+int pic_solver_solve(pic_solver *s, double *residual, int *iters);
+/* solveQN() with the solver's reference values (ch3/ver2/PotentialSolver.cpp:204-222). */
+int pic_solver_solve_qn(pic_solver *s);
+/* computeEF() (ch3/ver2/PotentialSolver.cpp:465-504). */
+int pic_solver_compute_ef(pic_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

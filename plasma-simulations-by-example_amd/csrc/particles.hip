@@ -1,0 +1,1927 @@
+// Species: particle storage in tile bins, fused push + gather + deposit + rebin, density
+// finish, counting sort, diagnostics, device injection.
+//
+// Layout.  Particles are structure-of-arrays, grouped in per-tile bins (tile = TILE^3 cells,
+// tiles numbered x-fastest).  Every push reads bins of buffer `cur` and appends each surviving
+// particle to the bin of its new tile in the other buffer, so bins stay exact every step,
+// dead particles vanish (the compaction of ch3/ver2/Species.cpp:36-46), and each workgroup
+// sees only particles inside its own tile: its E tile lives in LDS and its deposits land in an
+// LDS window of the tile plus a HALO-cell rim, written out once per tile (no global atomics
+// except for particles that jump further than the rim).
+#include <algorithm>
+#include <type_traits>
+#include <cmath>
+#include <cstring>
+
+#include "picamd_internal.h"
+
+namespace pic {
+
+int compute_ef(pic_world *w, bool want32, bool want64);
+
+constexpr int NSLOT_DIAG = 5;
+
+// ---------------------------------------------------------------------------------------
+// push parameters
+// ---------------------------------------------------------------------------------------
+struct PushParams {
+    Grid g;
+    int bc;
+    int sphere_on;
+    int remap;  // XCD-aware block -> tile remap
+    // PIC_MIXED (FP32, cell-relative units)
+    float qm_dt;     // dt*q/m
+    float dt_dh[3];  // dt/dh
+    int sph_c[3];    // sphere centre in cells: integer part
+    float sph_f[3];  //                          fractional part
+    float dhf[3];
+    float r2f;
+    // PIC_FP64 (physical units, the reference's arithmetic)
+    double s64;  // dt*charge/mass, as ch2/Species.cpp:26
+    double dt;
+    double sph_c64[3];
+    double sph_r2;
+};
+
+__device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+// block index -> tile index.  Blocks are dispatched round-robin over the 8 XCDs; giving
+// each XCD a contiguous run of tiles keeps neighbouring E tiles in the same L2.  Only used
+// when ntiles is a multiple of 8 (then it is a bijection on [0, ntiles)).
+__device__ inline int tile_of_block(int b, int ntiles, int remap) {
+    if (!remap) return b;
+    int per = ntiles >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// Ranks lanes that share a key within the wavefront (keys >= 0), reserving space for each
+// distinct key once per wavefront with one LDS atomic.  Returns the rank of this lane in
+// the block-wide counter of its key.
+__device__ inline uint32_t wave_reserve(int key, uint32_t *counter) {
+    uint32_t rank = 0;
+    unsigned long long active = __ballot(key >= 0);
+    while (active) {
+        int leader = __ffsll((long long)active) - 1;
+        int lk = __shfl(key, leader, 64);
+        unsigned long long m = __ballot(key == lk);
+        uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        uint32_t base = 0;
+        if ((int)__lane_id() == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (key == lk) rank = base + before;
+        active &= ~m;
+    }
+    return rank;
+}
+
+// Same for global per-tile counters (far movers, staging pushes, sort).
+__device__ inline uint32_t wave_reserve_global(int key, uint32_t *counter) {
+    uint32_t rank = 0;
+    unsigned long long active = __ballot(key >= 0);
+    while (active) {
+        int leader = __ffsll((long long)active) - 1;
+        int lk = __shfl(key, leader, 64);
+        unsigned long long m = __ballot(key == lk);
+        uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        uint32_t base = 0;
+        if ((int)__lane_id() == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (key == lk) rank = base + before;
+        active &= ~m;
+    }
+    return rank;
+}
+
+__device__ inline unsigned long long wave_count_u32(int key, uint32_t *counter) {
+    // histogram: one atomic per distinct key per wavefront
+    unsigned long long active = __ballot(key >= 0);
+    while (active) {
+        int leader = __ffsll((long long)active) - 1;
+        int lk = __shfl(key, leader, 64);
+        unsigned long long m = __ballot(key == lk);
+        if ((int)__lane_id() == leader) atomicAdd(&counter[lk], (uint32_t)__popcll(m));
+        active &= ~m;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// particle state in registers
+// ---------------------------------------------------------------------------------------
+struct PMixed {
+    int ci, cj, ck;
+    float ox, oy, oz, vx, vy, vz, w;
+    uint32_t id;
+};
+
+struct PF64 {
+    double x, y, z, vx, vy, vz, w;
+    uint32_t id;
+};
+
+__device__ inline void load_p(const PartView &v, long idx, PMixed &p) {
+    int c = v.cell[idx];
+    p.ci = cell_i(c);
+    p.cj = cell_j(c);
+    p.ck = cell_k(c);
+    p.ox = v.ox[idx];
+    p.oy = v.oy[idx];
+    p.oz = v.oz[idx];
+    p.vx = v.vx[idx];
+    p.vy = v.vy[idx];
+    p.vz = v.vz[idx];
+    p.w = v.w[idx];
+    p.id = v.id ? v.id[idx] : 0u;
+}
+
+__device__ inline void store_p(const PartView &v, long idx, const PMixed &p) {
+    v.cell[idx] = pack_cell(p.ci, p.cj, p.ck);
+    v.ox[idx] = p.ox;
+    v.oy[idx] = p.oy;
+    v.oz[idx] = p.oz;
+    v.vx[idx] = p.vx;
+    v.vy[idx] = p.vy;
+    v.vz[idx] = p.vz;
+    v.w[idx] = p.w;
+    if (v.id) v.id[idx] = p.id;
+}
+
+__device__ inline void load_p(const PartView &v, long idx, PF64 &p) {
+    p.x = v.x[idx];
+    p.y = v.y[idx];
+    p.z = v.z[idx];
+    p.vx = v.dvx[idx];
+    p.vy = v.dvy[idx];
+    p.vz = v.dvz[idx];
+    p.w = v.dw[idx];
+    p.id = v.id ? v.id[idx] : 0u;
+}
+
+__device__ inline void store_p(const PartView &v, long idx, const PF64 &p) {
+    v.x[idx] = p.x;
+    v.y[idx] = p.y;
+    v.z[idx] = p.z;
+    v.dvx[idx] = p.vx;
+    v.dvy[idx] = p.vy;
+    v.dvz[idx] = p.vz;
+    v.dw[idx] = p.w;
+    if (v.id) v.id[idx] = p.id;
+}
+
+// cell of an FP64 particle: i = (int)((x-x0)/dh), as World::XtoL + Field_::scatter
+// (ch2/World.h:72-78, Field.h:167-171); clamped so that a position that rounds onto the
+// upper wall never indexes past the last cell.
+__device__ inline void cell_of(const Grid &g, const PF64 &p, int &i, int &j, int &k, double &di,
+                               double &dj, double &dk) {
+#pragma clang fp contract(off)
+    double lx = (p.x - g.x0[0]) / g.dh[0];
+    double ly = (p.y - g.x0[1]) / g.dh[1];
+    double lz = (p.z - g.x0[2]) / g.dh[2];
+    i = (int)lx;
+    j = (int)ly;
+    k = (int)lz;
+    i = min(max(i, 0), g.nci - 1);
+    j = min(max(j, 0), g.ncj - 1);
+    k = min(max(k, 0), g.nck - 1);
+    di = lx - i;
+    dj = ly - j;
+    dk = lz - k;
+}
+
+__device__ inline int tile_id(const Grid &g, int ci, int cj, int ck) {
+    return (ci >> TILE_SHIFT) + g.ntx * ((cj >> TILE_SHIFT) + g.nty * (ck >> TILE_SHIFT));
+}
+
+__device__ inline int tile_of(const Grid &g, const PMixed &p) { return tile_id(g, p.ci, p.cj, p.ck); }
+
+__device__ inline int tile_of(const Grid &g, const PF64 &p) {
+    int i, j, k;
+    double a, b, c;
+    cell_of(g, p, i, j, k, a, b, c);
+    return tile_id(g, i, j, k);
+}
+
+// ---- FP32 push in cell-relative coordinates -------------------------------------------
+// Advances one axis: offset o in [0,1) of cell c moves by d cells.
+__device__ inline void advance_axis(int &c, float &o, float d) {
+    float n = o + d;
+    float fl = floorf(n);
+    n -= fl;
+    int s = (int)fl;
+    if (n >= 1.0f) {  // o+d-floor rounded up to 1
+        n = 0.0f;
+        s += 1;
+    }
+    c += s;
+    o = n;
+}
+
+// Specular wall on one axis with nc cells (ch2/Species.cpp:32-36 in logical units):
+// L < 0 -> -L, L >= nc -> 2nc - L, velocity reversed.  A particle that lands exactly on the
+// upper wall (L == nc after reflection, where the reference scatters past the array) is
+// clamped just inside it.
+__device__ inline void reflect_axis(int &c, float &o, float &v, int nc) {
+    if (c < 0) {
+        c = -c - 1;
+        o = 1.0f - o;
+        if (o >= 1.0f) {
+            o = 0.0f;
+            c += 1;
+        }
+        v = -v;
+    } else if (c >= nc) {
+        c = 2 * nc - c - 1;
+        o = 1.0f - o;
+        if (o >= 1.0f) {
+            o = 0.0f;
+            c += 1;
+        }
+        v = -v;
+    }
+    if (c >= nc) {
+        c = nc - 1;
+        o = 0x1.fffffep-1f;
+    }
+    if (c < 0) {  // moved more than the box width in one step: keep memory-safe
+        c = 0;
+        o = 0.0f;
+    }
+}
+
+// E at a particle from an LDS E tile (float4 per node, TN^3 nodes), trilinear (CIC).
+__device__ inline float3 gather_lds(const float4 *sE, int li, int lj, int lk, float ox, float oy,
+                                    float oz) {
+    int b = li + TN * (lj + TN * lk);
+    float4 e000 = sE[b], e100 = sE[b + 1], e010 = sE[b + TN], e110 = sE[b + TN + 1];
+    float4 e001 = sE[b + TN * TN], e101 = sE[b + TN * TN + 1], e011 = sE[b + TN * TN + TN],
+           e111 = sE[b + TN * TN + TN + 1];
+    float ax = 1.0f - ox, ay = 1.0f - oy, az = 1.0f - oz;
+    float w000 = ax * ay * az, w100 = ox * ay * az, w010 = ax * oy * az, w110 = ox * oy * az;
+    float w001 = ax * ay * oz, w101 = ox * ay * oz, w011 = ax * oy * oz, w111 = ox * oy * oz;
+    float3 E;
+    E.x = e000.x * w000 + e100.x * w100 + e110.x * w110 + e010.x * w010 + e001.x * w001 +
+          e101.x * w101 + e111.x * w111 + e011.x * w011;
+    E.y = e000.y * w000 + e100.y * w100 + e110.y * w110 + e010.y * w010 + e001.y * w001 +
+          e101.y * w101 + e111.y * w111 + e011.y * w011;
+    E.z = e000.z * w000 + e100.z * w100 + e110.z * w110 + e010.z * w010 + e001.z * w001 +
+          e101.z * w101 + e111.z * w111 + e011.z * w011;
+    return E;
+}
+
+__device__ inline float3 gather_global32(const Grid &g, const float4 *ef, int ci, int cj, int ck,
+                                         float ox, float oy, float oz) {
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    long b = ci + sj * cj + sk * ck;
+    float4 e000 = ef[b], e100 = ef[b + 1], e010 = ef[b + sj], e110 = ef[b + sj + 1];
+    float4 e001 = ef[b + sk], e101 = ef[b + sk + 1], e011 = ef[b + sk + sj], e111 = ef[b + sk + sj + 1];
+    float ax = 1.0f - ox, ay = 1.0f - oy, az = 1.0f - oz;
+    float w000 = ax * ay * az, w100 = ox * ay * az, w010 = ax * oy * az, w110 = ox * oy * az;
+    float w001 = ax * ay * oz, w101 = ox * ay * oz, w011 = ax * oy * oz, w111 = ox * oy * oz;
+    float3 E;
+    E.x = e000.x * w000 + e100.x * w100 + e110.x * w110 + e010.x * w010 + e001.x * w001 +
+          e101.x * w101 + e111.x * w111 + e011.x * w011;
+    E.y = e000.y * w000 + e100.y * w100 + e110.y * w110 + e010.y * w010 + e001.y * w001 +
+          e101.y * w101 + e111.y * w111 + e011.y * w011;
+    E.z = e000.z * w000 + e100.z * w100 + e110.z * w110 + e010.z * w010 + e001.z * w001 +
+          e101.z * w101 + e111.z * w111 + e011.z * w011;
+    return E;
+}
+
+// Leapfrog step of a PIC_MIXED particle given E: v += E*(dt q/m); x += v dt, then the wall
+// treatment.  Returns false when the particle is absorbed.
+__device__ inline bool push_mixed(const PushParams &P, PMixed &p, float3 E) {
+    p.vx += E.x * P.qm_dt;
+    p.vy += E.y * P.qm_dt;
+    p.vz += E.z * P.qm_dt;
+    advance_axis(p.ci, p.ox, p.vx * P.dt_dh[0]);
+    advance_axis(p.cj, p.oy, p.vy * P.dt_dh[1]);
+    advance_axis(p.ck, p.oz, p.vz * P.dt_dh[2]);
+    if (P.bc == PIC_BC_REFLECT) {
+        reflect_axis(p.ci, p.ox, p.vx, P.g.nci);
+        reflect_axis(p.cj, p.oy, p.vy, P.g.ncj);
+        reflect_axis(p.ck, p.oz, p.vz, P.g.nck);
+        return true;
+    }
+    // absorbing: !inBounds (half-open [x0,xm)) || inSphere  (ch3/ver2/Species.cpp:28)
+    if (p.ci < 0 || p.ci >= P.g.nci || p.cj < 0 || p.cj >= P.g.ncj || p.ck < 0 || p.ck >= P.g.nck)
+        return false;
+    if (P.sphere_on) {
+        float rx = ((float)(p.ci - P.sph_c[0]) + (p.ox - P.sph_f[0])) * P.dhf[0];
+        float ry = ((float)(p.cj - P.sph_c[1]) + (p.oy - P.sph_f[1])) * P.dhf[1];
+        float rz = ((float)(p.ck - P.sph_c[2]) + (p.oz - P.sph_f[2])) * P.dhf[2];
+        if (rx * rx + ry * ry + rz * rz <= P.r2f) return false;
+    }
+    return true;
+}
+
+// ---- FP64 push: the reference's arithmetic (ch2/Species.cpp:19-36, ch3/ver2/Species.cpp) --
+__device__ inline double3 gather64_weights(const double *Ex, const double *Ey, const double *Ez,
+                                           long b, long sj, long sk, double di, double dj,
+                                           double dk) {
+#pragma clang fp contract(off)
+    // Field_::gather (ch2/Field.h:186-208): corners in the reference's order, each term
+    // E*(w_i)*(w_j)*(w_k), summed left to right.
+    const long off[8] = {0, 1, 1 + sj, sj, sk, 1 + sk, 1 + sj + sk, sj + sk};
+    const double wi[8] = {1 - di, di, di, 1 - di, 1 - di, di, di, 1 - di};
+    const double wj[8] = {1 - dj, 1 - dj, dj, dj, 1 - dj, 1 - dj, dj, dj};
+    const double wk[8] = {1 - dk, 1 - dk, 1 - dk, 1 - dk, dk, dk, dk, dk};
+    double3 r;
+    r.x = Ex[b + off[0]] * wi[0] * wj[0] * wk[0];
+    r.y = Ey[b + off[0]] * wi[0] * wj[0] * wk[0];
+    r.z = Ez[b + off[0]] * wi[0] * wj[0] * wk[0];
+#pragma unroll
+    for (int n = 1; n < 8; n++) {
+        r.x = r.x + Ex[b + off[n]] * wi[n] * wj[n] * wk[n];
+        r.y = r.y + Ey[b + off[n]] * wi[n] * wj[n] * wk[n];
+        r.z = r.z + Ez[b + off[n]] * wi[n] * wj[n] * wk[n];
+    }
+    return r;
+}
+
+__device__ inline double3 gather64_global(const double4 *ef, long b, long sj, long sk, double di,
+                                          double dj, double dk) {
+#pragma clang fp contract(off)
+    const long off[8] = {0, 1, 1 + sj, sj, sk, 1 + sk, 1 + sj + sk, sj + sk};
+    const double wi[8] = {1 - di, di, di, 1 - di, 1 - di, di, di, 1 - di};
+    const double wj[8] = {1 - dj, 1 - dj, dj, dj, 1 - dj, 1 - dj, dj, dj};
+    const double wk[8] = {1 - dk, 1 - dk, 1 - dk, 1 - dk, dk, dk, dk, dk};
+    double4 e = ef[b];
+    double3 r;
+    r.x = e.x * wi[0] * wj[0] * wk[0];
+    r.y = e.y * wi[0] * wj[0] * wk[0];
+    r.z = e.z * wi[0] * wj[0] * wk[0];
+#pragma unroll
+    for (int n = 1; n < 8; n++) {
+        e = ef[b + off[n]];
+        r.x = r.x + e.x * wi[n] * wj[n] * wk[n];
+        r.y = r.y + e.y * wi[n] * wj[n] * wk[n];
+        r.z = r.z + e.z * wi[n] * wj[n] * wk[n];
+    }
+    return r;
+}
+
+__device__ inline bool push_f64(const PushParams &P, PF64 &p, double3 E) {
+#pragma clang fp contract(off)
+    const Grid &g = P.g;
+    p.vx += E.x * P.s64;
+    p.vy += E.y * P.s64;
+    p.vz += E.z * P.s64;
+    p.x += p.vx * P.dt;
+    p.y += p.vy * P.dt;
+    p.z += p.vz * P.dt;
+    if (P.bc == PIC_BC_REFLECT) {
+        double *xs[3] = {&p.x, &p.y, &p.z};
+        double *vs[3] = {&p.vx, &p.vy, &p.vz};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double &x = *xs[c];
+            if (x < g.x0[c]) {
+                x = 2 * g.x0[c] - x;
+                *vs[c] *= -1.0;
+            } else if (x >= g.xm[c]) {
+                x = 2 * g.xm[c] - x;
+                *vs[c] *= -1.0;
+            }
+        }
+        return true;
+    }
+    if (p.x < g.x0[0] || p.x >= g.xm[0] || p.y < g.x0[1] || p.y >= g.xm[1] || p.z < g.x0[2] ||
+        p.z >= g.xm[2])
+        return false;
+    if (P.sphere_on) {
+        double r0 = p.x - P.sph_c64[0], r1 = p.y - P.sph_c64[1], r2 = p.z - P.sph_c64[2];
+        double m2 = (r0 * r0 + r1 * r1 + r2 * r2);
+        if (m2 <= P.sph_r2) return false;
+    }
+    return true;
+}
+
+// ---- deposit helpers --------------------------------------------------------------------
+__device__ inline void deposit_global64(const Grid &g, double *den, int i, int j, int k, double di,
+                                        double dj, double dk, double w) {
+#pragma clang fp contract(off)
+    // Field_::scatter (ch2/Field.h:164-183)
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    long b = i + sj * j + sk * k;
+    atomicAdd(&den[b], w * (1 - di) * (1 - dj) * (1 - dk));
+    atomicAdd(&den[b + 1], w * (di) * (1 - dj) * (1 - dk));
+    atomicAdd(&den[b + 1 + sj], w * (di) * (dj) * (1 - dk));
+    atomicAdd(&den[b + sj], w * (1 - di) * (dj) * (1 - dk));
+    atomicAdd(&den[b + sk], w * (1 - di) * (1 - dj) * (dk));
+    atomicAdd(&den[b + 1 + sk], w * (di) * (1 - dj) * (dk));
+    atomicAdd(&den[b + 1 + sj + sk], w * (di) * (dj) * (dk));
+    atomicAdd(&den[b + sj + sk], w * (1 - di) * (dj) * (dk));
+}
+
+template <typename T>
+__device__ inline void deposit_lds(T *sW, int wi, int wj, int wk, T ox, T oy, T oz, T w) {
+    int b = wi + WN * (wj + WN * wk);
+    T ax = T(1) - ox, ay = T(1) - oy, az = T(1) - oz;
+    atomicAdd(&sW[b], w * ax * ay * az);
+    atomicAdd(&sW[b + 1], w * ox * ay * az);
+    atomicAdd(&sW[b + 1 + WN], w * ox * oy * az);
+    atomicAdd(&sW[b + WN], w * ax * oy * az);
+    atomicAdd(&sW[b + WN * WN], w * ax * ay * oz);
+    atomicAdd(&sW[b + 1 + WN * WN], w * ox * ay * oz);
+    atomicAdd(&sW[b + 1 + WN + WN * WN], w * ox * oy * oz);
+    atomicAdd(&sW[b + WN + WN * WN], w * ax * oy * oz);
+}
+
+// ---------------------------------------------------------------------------------------
+// per-step bin plan: capacities of the output bins from the input counts
+// ---------------------------------------------------------------------------------------
+// One workgroup of 1024 threads.  cap[t] = need[t] + slack[t], slack = max(n/8, nb/4) + 32
+// where nb is the largest face-neighbour population (room for one step of inflow); the
+// slack is scaled down uniformly when the buffer is short.  Writes starts (exclusive scan),
+// zeroes the output counts and the step's counters.
+constexpr int PLAN_THREADS = 1024;
+
+__device__ inline uint32_t clamped_count(const uint32_t *cnt, const uint32_t *cap, int t) {
+    uint32_t c = cnt[t], k = cap[t];
+    return c < k ? c : k;
+}
+
+__global__ void __launch_bounds__(PLAN_THREADS)
+    k_plan(Grid g, const uint32_t *cnt_in, const uint32_t *cap_in, const uint32_t *hist, int mode,
+           uint32_t *start_out, uint32_t *cap_out, uint32_t *count_out, uint32_t *stage_count_out,
+           long cap_bins, long *np_dev, double *diag, uint32_t *err) {
+    __shared__ unsigned long long s_need[PLAN_THREADS / 64], s_slack[PLAN_THREADS / 64];
+    __shared__ unsigned long long s_scan[PLAN_THREADS];
+    int nt = g.ntiles;
+    int per = (nt + PLAN_THREADS - 1) / PLAN_THREADS;
+    int t0 = threadIdx.x * per, t1 = min(nt, t0 + per);
+    unsigned long long need = 0, slack = 0;
+    for (int t = t0; t < t1; t++) {
+        uint32_t n, nb = 0;
+        if (mode == 0) {
+            n = clamped_count(cnt_in, cap_in, t);
+            int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+            if (tx > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - 1));
+            if (tx < g.ntx - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + 1));
+            if (ty > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx));
+            if (ty < g.nty - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx));
+            if (tz > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx * g.nty));
+            if (tz < g.ntz - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx * g.nty));
+        } else {
+            n = hist[t];
+        }
+        need += n;
+        uint32_t s = mode == 0 ? max(n / 8, nb / 4) + 32 : n / 8 + 32;
+        slack += s;
+    }
+    // block sums
+    unsigned long long a = need, b = slack;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_need[threadIdx.x >> 6] = a;
+        s_slack[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    unsigned long long tot_need = 0, tot_slack = 0;
+    for (int i = 0; i < PLAN_THREADS / 64; i++) {
+        tot_need += s_need[i];
+        tot_slack += s_slack[i];
+    }
+    double alpha = 1.0;
+    if (tot_need > (unsigned long long)cap_bins) {
+        alpha = 0.0;
+        if (threadIdx.x == 0) atomicOr(err, 1u);  // caller grows the buffers
+    } else if (tot_need + tot_slack > (unsigned long long)cap_bins) {
+        alpha = (double)((unsigned long long)cap_bins - tot_need) / (double)tot_slack;
+    }
+    // caps and per-thread sums
+    unsigned long long local = 0;
+    for (int t = t0; t < t1; t++) {
+        uint32_t n, nb = 0;
+        if (mode == 0) {
+            n = clamped_count(cnt_in, cap_in, t);
+            int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+            if (tx > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - 1));
+            if (tx < g.ntx - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + 1));
+            if (ty > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx));
+            if (ty < g.nty - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx));
+            if (tz > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx * g.nty));
+            if (tz < g.ntz - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx * g.nty));
+        } else {
+            n = hist[t];
+        }
+        uint32_t s = mode == 0 ? max(n / 8, nb / 4) + 32 : n / 8 + 32;
+        uint32_t c = n + (uint32_t)(alpha * (double)s);
+        if (tot_need > (unsigned long long)cap_bins) c = 0;
+        cap_out[t] = c;
+        local += c;
+    }
+    // exclusive scan of the per-thread sums (Hillis-Steele over 1024 entries)
+    s_scan[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < PLAN_THREADS; off <<= 1) {
+        unsigned long long v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = s_scan[threadIdx.x] - local;
+    for (int t = t0; t < t1; t++) {
+        start_out[t] = (uint32_t)run;
+        run += cap_out[t];
+        count_out[t] = 0;
+    }
+    if (threadIdx.x == 0) {
+        *stage_count_out = 0;
+        if (np_dev) *np_dev = 0;
+        if (diag)
+            for (int q = 0; q < NSLOT_DIAG; q++) diag[q] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// the fused push: gather (LDS E tile) + leapfrog + walls + CIC deposit (LDS window) + rebin
+// ---------------------------------------------------------------------------------------
+struct PushOut {
+    float *win32;
+    double *win64;
+    uint8_t *win_used;
+    double *den_extra;
+    long *np_dev;
+    double *diag;
+    uint32_t *err;
+};
+
+template <typename P>
+__device__ inline void append_particle(const Grid &g, const PartView &out, const P &p, int dtile,
+                                       uint32_t pos, const PushOut &O) {
+    long idx;
+    if (pos < out.bin_cap[dtile]) {
+        idx = (long)out.bin_start[dtile] + pos;
+    } else {
+        uint32_t s = atomicAdd(out.stage_count, 1u);
+        if (s >= out.cap_stage) {
+            atomicOr(O.err, 2u);  // staging full: particle dropped, reported as capacity error
+            return;
+        }
+        idx = out.cap_bins + s;
+    }
+    store_p(out, idx, p);
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(PUSH_THREADS)
+    k_push(PushParams P, PartView in, PartView out, const float4 *__restrict__ ef32,
+           const double4 *__restrict__ ef64, PushOut O) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    using WT = typename std::conditional<PREC == PIC_MIXED, float, double>::type;
+    const Grid &g = P.g;
+    __shared__ uint32_t sCnt[NDEST + 1], sBase[NDEST + 1];
+    __shared__ double sRed[NSLOT_DIAG + 1][PUSH_THREADS / 64];
+    // LDS: E tile (float4 per node, or 3 double planes) and the deposit window
+    __shared__ __attribute__((aligned(16))) unsigned char
+        sRaw[(PREC == PIC_MIXED ? TNODES * 16 : TNODES * 24)];
+    __shared__ WT sW[WNODES];
+
+    double acc[NSLOT_DIAG] = {0, 0, 0, 0, 0};
+    double alive_count = 0;
+
+    const int tid = threadIdx.x;
+    if ((int)blockIdx.x < g.ntiles) {
+        // ------------------------------------------------ binned tile
+        int t = tile_of_block(blockIdx.x, g.ntiles, P.remap);
+        uint32_t cnt = clamped_count(in.bin_count, in.bin_cap, t);
+        if (cnt == 0) {
+            if (tid == 0) O.win_used[t] = 0;
+            return;
+        }
+        uint32_t start = in.bin_start[t];
+        int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+        int c0x = tx * TILE, c0y = ty * TILE, c0z = tz * TILE;
+        long sj = g.ni, sk = (long)g.ni * g.nj;
+        // load the E tile
+        for (int n = tid; n < TNODES; n += PUSH_THREADS) {
+            int a = n % TN, b = (n / TN) % TN, c = n / (TN * TN);
+            int gi = c0x + a, gj = c0y + b, gk = c0z + c;
+            bool ok = gi < g.ni && gj < g.nj && gk < g.nk;
+            long u = gi + sj * gj + sk * gk;
+            if (PREC == PIC_MIXED) {
+                float4 e = ok ? ef32[u] : make_float4(0.f, 0.f, 0.f, 0.f);
+                reinterpret_cast<float4 *>(sRaw)[n] = e;
+            } else {
+                double4 e = ok ? ef64[u] : make_double4(0, 0, 0, 0);
+                double *pl = reinterpret_cast<double *>(sRaw);
+                pl[n] = e.x;
+                pl[TNODES + n] = e.y;
+                pl[2 * TNODES + n] = e.z;
+            }
+        }
+        for (int n = tid; n < WNODES; n += PUSH_THREADS) sW[n] = WT(0);
+        if (tid <= NDEST) sCnt[tid] = 0;
+        __syncthreads();
+
+        for (uint32_t base = 0; base < cnt; base += PUSH_BATCH) {
+            PT pp[PUSH_PER_THREAD];
+            int dest[PUSH_PER_THREAD], dtile[PUSH_PER_THREAD];
+            uint32_t rank[PUSH_PER_THREAD];
+#pragma unroll
+            for (int r = 0; r < PUSH_PER_THREAD; r++) {
+                uint32_t q = base + r * PUSH_THREADS + tid;
+                dest[r] = -1;
+                dtile[r] = -1;
+                if (q < cnt) {
+                    PT &p = pp[r];
+                    load_p(in, (long)start + q, p);
+                    bool alive;
+                    int ni_, nj_, nk_;
+                    if constexpr (PREC == PIC_MIXED) {
+                        int li = min(max(p.ci - c0x, 0), TILE - 1);
+                        int lj = min(max(p.cj - c0y, 0), TILE - 1);
+                        int lk = min(max(p.ck - c0z, 0), TILE - 1);
+                        float3 E = gather_lds(reinterpret_cast<const float4 *>(sRaw), li, lj, lk,
+                                              p.ox, p.oy, p.oz);
+                        alive = push_mixed(P, p, E);
+                        ni_ = p.ci;
+                        nj_ = p.cj;
+                        nk_ = p.ck;
+                        if (alive) {
+                            int wi = p.ci - c0x + HALO, wj = p.cj - c0y + HALO, wk = p.ck - c0z + HALO;
+                            if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO &&
+                                wk >= 0 && wk < TILE + 2 * HALO)
+                                deposit_lds<float>(reinterpret_cast<float *>(sW), wi, wj, wk, p.ox,
+                                                   p.oy, p.oz, p.w);
+                            else
+                                deposit_global64(g, O.den_extra, p.ci, p.cj, p.ck, (double)p.ox,
+                                                 (double)p.oy, (double)p.oz, (double)p.w);
+                        }
+                    } else {
+                        int i, j, k;
+                        double di, dj, dk;
+                        cell_of(g, p, i, j, k, di, dj, dk);
+                        int li = min(max(i - c0x, 0), TILE - 1);
+                        int lj = min(max(j - c0y, 0), TILE - 1);
+                        int lk = min(max(k - c0z, 0), TILE - 1);
+                        const double *pl = reinterpret_cast<const double *>(sRaw);
+                        double3 E = gather64_weights(pl, pl + TNODES, pl + 2 * TNODES,
+                                                     li + TN * (lj + TN * lk), TN, TN * TN, di, dj, dk);
+                        alive = push_f64(P, p, E);
+                        if (alive) {
+                            cell_of(g, p, i, j, k, di, dj, dk);
+                            ni_ = i;
+                            nj_ = j;
+                            nk_ = k;
+                            int wi = i - c0x + HALO, wj = j - c0y + HALO, wk = k - c0z + HALO;
+                            if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO &&
+                                wk >= 0 && wk < TILE + 2 * HALO)
+                                deposit_lds<double>(reinterpret_cast<double *>(sW), wi, wj, wk, di, dj,
+                                                    dk, p.w);
+                            else
+                                deposit_global64(g, O.den_extra, i, j, k, di, dj, dk, p.w);
+                        }
+                    }
+                    if (alive) {
+                        double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
+                        acc[0] += w;
+                        acc[1] += vx * w;
+                        acc[2] += vy * w;
+                        acc[3] += vz * w;
+                        acc[4] += w * (vx * vx + vy * vy + vz * vz);
+                        alive_count += 1;
+                        int ntx_ = ni_ >> TILE_SHIFT, nty_ = nj_ >> TILE_SHIFT, ntz_ = nk_ >> TILE_SHIFT;
+                        int dx = ntx_ - tx, dy = nty_ - ty, dz = ntz_ - tz;
+                        dtile[r] = ntx_ + g.ntx * (nty_ + g.nty * ntz_);
+                        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
+                            dest[r] = (dx + 1) + 3 * ((dy + 1) + 3 * (dz + 1));
+                        else
+                            dest[r] = NDEST;  // far mover
+                    }
+                }
+            }
+            // reserve slots: wave-aggregated LDS counters per destination tile
+#pragma unroll
+            for (int r = 0; r < PUSH_PER_THREAD; r++) {
+                int key = (dest[r] >= 0 && dest[r] < NDEST) ? dest[r] : -1;
+                rank[r] = wave_reserve(key, sCnt);
+            }
+            __syncthreads();
+            if (tid < NDEST && sCnt[tid] > 0) {
+                int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
+                int dt = (tx + dx) + g.ntx * ((ty + dy) + g.nty * (tz + dz));
+                sBase[tid] = atomicAdd(&out.bin_count[dt], sCnt[tid]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < PUSH_PER_THREAD; r++) {
+                if (dest[r] < 0) continue;
+                if (dest[r] < NDEST) {
+                    append_particle(g, out, pp[r], dtile[r], sBase[dest[r]] + rank[r], O);
+                } else {
+                    uint32_t pos = atomicAdd(&out.bin_count[dtile[r]], 1u);
+                    append_particle(g, out, pp[r], dtile[r], pos, O);
+                }
+            }
+            __syncthreads();
+            if (tid <= NDEST) sCnt[tid] = 0;
+            __syncthreads();
+        }
+        // flush the deposit window of this tile
+        if (PREC == PIC_MIXED) {
+            float *dst = O.win32 + (long)t * WNODES;
+            for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = reinterpret_cast<float *>(sW)[n];
+        } else {
+            double *dst = O.win64 + (long)t * WNODES;
+            for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = reinterpret_cast<double *>(sW)[n];
+        }
+        if (tid == 0) O.win_used[t] = 1;
+    } else {
+        // ------------------------------------------------ unbinned staging list
+        uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
+        uint32_t nthreads = (gridDim.x - g.ntiles) * PUSH_THREADS;
+        uint32_t first = (blockIdx.x - g.ntiles) * PUSH_THREADS + tid;
+        // uniform trip count per wavefront so that wave_reserve sees all lanes
+        uint32_t trips = (ns + nthreads - 1) / nthreads;
+        for (uint32_t it = 0; it < trips; it++) {
+            uint32_t q = first + it * nthreads;
+            PT p;
+            int dt = -1;
+            if (q < ns) {
+                load_p(in, in.cap_bins + q, p);
+                bool alive;
+                if constexpr (PREC == PIC_MIXED) {
+                    float3 E = gather_global32(g, ef32, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz);
+                    alive = push_mixed(P, p, E);
+                    if (alive) {
+                        deposit_global64(g, O.den_extra, p.ci, p.cj, p.ck, (double)p.ox, (double)p.oy,
+                                         (double)p.oz, (double)p.w);
+                        dt = tile_of(g, p);
+                    }
+                } else {
+                    int i, j, k;
+                    double di, dj, dk;
+                    cell_of(g, p, i, j, k, di, dj, dk);
+                    long sj = g.ni, sk = (long)g.ni * g.nj;
+                    double3 E = gather64_global(ef64, i + sj * j + sk * k, sj, sk, di, dj, dk);
+                    alive = push_f64(P, p, E);
+                    if (alive) {
+                        cell_of(g, p, i, j, k, di, dj, dk);
+                        deposit_global64(g, O.den_extra, i, j, k, di, dj, dk, p.w);
+                        dt = tile_id(g, i, j, k);
+                    }
+                }
+                if (alive) {
+                    double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
+                    acc[0] += w;
+                    acc[1] += vx * w;
+                    acc[2] += vy * w;
+                    acc[3] += vz * w;
+                    acc[4] += w * (vx * vx + vy * vy + vz * vz);
+                    alive_count += 1;
+                }
+            }
+            uint32_t pos = wave_reserve_global(dt, out.bin_count);
+            if (dt >= 0) append_particle(g, out, p, dt, pos, O);
+        }
+    }
+    // diagnostics of the pushed particles (getRealCount, getMomentum, getKE) and np
+#pragma unroll
+    for (int q = 0; q < NSLOT_DIAG; q++) {
+        double v = warp_sum(acc[q]);
+        if ((tid & 63) == 0) sRed[q][tid >> 6] = v;
+    }
+    {
+        double v = warp_sum(alive_count);
+        if ((tid & 63) == 0) sRed[NSLOT_DIAG][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < NSLOT_DIAG + 1) {
+        double s = 0;
+        for (int i = 0; i < PUSH_THREADS / 64; i++) s += sRed[tid][i];
+        if (tid < NSLOT_DIAG) {
+            if (s != 0) atomicAdd(&O.diag[tid], s);
+        } else if (s != 0) {
+            atomicAdd((unsigned long long *)O.np_dev, (unsigned long long)s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// density finish: den = (sum of tile windows + global extras) / node_vol
+// ---------------------------------------------------------------------------------------
+template <typename WT>
+__global__ void __launch_bounds__(256)
+    k_finish_density(Grid g, const WT *__restrict__ win, const uint8_t *__restrict__ win_used,
+                     double *den_extra, double *den) {
+#pragma clang fp contract(off)
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u >= g.nn) return;
+    int i = (int)(u % g.ni);
+    int j = (int)((u / g.ni) % g.nj);
+    int k = (int)(u / ((long)g.ni * g.nj));
+    double raw = den_extra[u];
+    den_extra[u] = 0;  // ready for the next deposit
+    // tiles whose window [a*TILE-HALO, a*TILE+TILE+HALO] holds node i
+    int ax0 = max(0, floordiv(i - TILE - HALO + TILE - 1, TILE)), ax1 = min(g.ntx - 1, floordiv(i + HALO, TILE));
+    int ay0 = max(0, floordiv(j - TILE - HALO + TILE - 1, TILE)), ay1 = min(g.nty - 1, floordiv(j + HALO, TILE));
+    int az0 = max(0, floordiv(k - TILE - HALO + TILE - 1, TILE)), az1 = min(g.ntz - 1, floordiv(k + HALO, TILE));
+    for (int c = az0; c <= az1; c++)
+        for (int b = ay0; b <= ay1; b++)
+            for (int a = ax0; a <= ax1; a++) {
+                int t = a + g.ntx * (b + g.nty * c);
+                if (!win_used[t]) continue;
+                int li = i - a * TILE + HALO, lj = j - b * TILE + HALO, lk = k - c * TILE + HALO;
+                raw += (double)win[(long)t * WNODES + li + WN * (lj + WN * lk)];
+            }
+    // den /= node_vol (ch2/Field.h:125-134; volumes as World::computeNodeVolumes)
+    double V = g.dh[0] * g.dh[1] * g.dh[2];
+    if (i == 0 || i == g.ni - 1) V *= 0.5;
+    if (j == 0 || j == g.nj - 1) V *= 0.5;
+    if (k == 0 || k == g.nk - 1) V *= 0.5;
+    den[u] = (V != 0) ? raw / V : 0.0;
+}
+
+// deposit-only pass (computeNumberDensity without a preceding push): global FP64 atomics.
+template <int PREC>
+__global__ void __launch_bounds__(256) k_deposit_all(Grid g, PartView in, double *den_extra) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    int b = blockIdx.x;
+    if (b < g.ntiles) {
+        uint32_t cnt = clamped_count(in.bin_count, in.bin_cap, b);
+        uint32_t start = in.bin_start[b];
+        for (uint32_t q = threadIdx.x; q < cnt; q += blockDim.x) {
+            PT p;
+            load_p(in, (long)start + q, p);
+            if constexpr (PREC == PIC_MIXED)
+                deposit_global64(g, den_extra, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz, p.w);
+            else {
+                int i, j, k;
+                double di, dj, dk;
+                cell_of(g, p, i, j, k, di, dj, dk);
+                deposit_global64(g, den_extra, i, j, k, di, dj, dk, p.w);
+            }
+        }
+    } else {
+        uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
+        for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns;
+             q += (gridDim.x - g.ntiles) * blockDim.x) {
+            PT p;
+            load_p(in, in.cap_bins + q, p);
+            if constexpr (PREC == PIC_MIXED)
+                deposit_global64(g, den_extra, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz, p.w);
+            else {
+                int i, j, k;
+                double di, dj, dk;
+                cell_of(g, p, i, j, k, di, dj, dk);
+                deposit_global64(g, den_extra, i, j, k, di, dj, dk, p.w);
+            }
+        }
+    }
+}
+
+// diagnostics over all particles (when the fused ones are stale)
+template <int PREC>
+__global__ void __launch_bounds__(256) k_diag_all(Grid g, PartView in, double *diag, long *np) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    __shared__ double sh[8];
+    double acc[NSLOT_DIAG] = {0, 0, 0, 0, 0};
+    double cnt_all = 0;
+    int b = blockIdx.x;
+    auto add = [&](const PT &p) {
+        double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
+        acc[0] += w;
+        acc[1] += vx * w;
+        acc[2] += vy * w;
+        acc[3] += vz * w;
+        acc[4] += w * (vx * vx + vy * vy + vz * vz);
+        cnt_all += 1;
+    };
+    if (b < g.ntiles) {
+        uint32_t cnt = clamped_count(in.bin_count, in.bin_cap, b);
+        uint32_t start = in.bin_start[b];
+        for (uint32_t q = threadIdx.x; q < cnt; q += blockDim.x) {
+            PT p;
+            load_p(in, (long)start + q, p);
+            add(p);
+        }
+    } else {
+        uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
+        for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns;
+             q += (gridDim.x - g.ntiles) * blockDim.x) {
+            PT p;
+            load_p(in, in.cap_bins + q, p);
+            add(p);
+        }
+    }
+    for (int q = 0; q < NSLOT_DIAG; q++) {
+        double s = block_sum(acc[q], sh);
+        if (threadIdx.x == 0 && s != 0) atomicAdd(&diag[q], s);
+    }
+    double s = block_sum(cnt_all, sh);
+    if (threadIdx.x == 0 && s != 0) atomicAdd((unsigned long long *)np, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------------------------------
+// counting sort: histogram -> plan (mode 1) -> scatter
+// ---------------------------------------------------------------------------------------
+template <int PREC>
+__global__ void __launch_bounds__(256) k_sort_hist(Grid g, PartView in, uint32_t *hist) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    int b = blockIdx.x;
+    uint32_t cnt, start;
+    long base;
+    uint32_t step, first;
+    if (b < g.ntiles) {
+        cnt = clamped_count(in.bin_count, in.bin_cap, b);
+        start = in.bin_start[b];
+        base = start;
+        first = threadIdx.x;
+        step = blockDim.x;
+    } else {
+        cnt = min(*in.stage_count, (uint32_t)in.cap_stage);
+        base = in.cap_bins;
+        first = (b - g.ntiles) * blockDim.x + threadIdx.x;
+        step = (gridDim.x - g.ntiles) * blockDim.x;
+    }
+    uint32_t trips = cnt > first - threadIdx.x ? (cnt - (first - threadIdx.x) + step - 1) / step : 0;
+    for (uint32_t it = 0; it < trips; it++) {
+        uint32_t q = first + it * step;
+        int key = -1;
+        if (q < cnt) {
+            PT p;
+            load_p(in, base + q, p);
+            key = tile_of(g, p);
+        }
+        wave_count_u32(key, hist);
+    }
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(256) k_sort_scatter(Grid g, PartView in, PartView out, uint32_t *err) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    int b = blockIdx.x;
+    uint32_t cnt;
+    long base;
+    uint32_t step, first;
+    if (b < g.ntiles) {
+        cnt = clamped_count(in.bin_count, in.bin_cap, b);
+        base = in.bin_start[b];
+        first = threadIdx.x;
+        step = blockDim.x;
+    } else {
+        cnt = min(*in.stage_count, (uint32_t)in.cap_stage);
+        base = in.cap_bins;
+        first = (b - g.ntiles) * blockDim.x + threadIdx.x;
+        step = (gridDim.x - g.ntiles) * blockDim.x;
+    }
+    uint32_t trips = cnt > first - threadIdx.x ? (cnt - (first - threadIdx.x) + step - 1) / step : 0;
+    for (uint32_t it = 0; it < trips; it++) {
+        uint32_t q = first + it * step;
+        int key = -1;
+        PT p;
+        if (q < cnt) {
+            load_p(in, base + q, p);
+            key = tile_of(g, p);
+        }
+        uint32_t pos = wave_reserve_global(key, out.bin_count);
+        if (key >= 0) {
+            if (pos < out.bin_cap[key])
+                store_p(out, (long)out.bin_start[key] + pos, p);
+            else
+                atomicOr(err, 4u);  // cannot happen: caps >= histogram
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// uploads, downloads, injection
+// ---------------------------------------------------------------------------------------
+// Converts FP64 host-format particles (interleaved pos/vel) into the species format and
+// appends them to the staging list.  rewind != 0 applies addParticle's half-step rewind
+// v -= (q/m)*E(x)*(0.5*dt) with the current E (ch2/Species.cpp:55-71); positions were
+// already bounds-checked on the host.
+template <int PREC>
+__global__ void __launch_bounds__(256)
+    k_append(Grid g, long n, const double *pos, const double *vel, const double *mpw, const uint32_t *ids,
+             uint32_t id0, int rewind, double qm, double half_dt, const float4 *ef32,
+             const double4 *ef64, PartView dst, uint32_t stage0) {
+#pragma clang fp contract(off)
+    long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double x = pos[3 * q], y = pos[3 * q + 1], z = pos[3 * q + 2];
+    double vx = vel[3 * q], vy = vel[3 * q + 1], vz = vel[3 * q + 2];
+    double lx = (x - g.x0[0]) / g.dh[0], ly = (y - g.x0[1]) / g.dh[1], lz = (z - g.x0[2]) / g.dh[2];
+    int i = min(max((int)lx, 0), g.nci - 1);
+    int j = min(max((int)ly, 0), g.ncj - 1);
+    int k = min(max((int)lz, 0), g.nck - 1);
+    double di = lx - i, dj = ly - j, dk = lz - k;
+    if (rewind) {
+        long sj = g.ni, sk = (long)g.ni * g.nj;
+        double3 E;
+        if (PREC == PIC_FP64) {
+            E = gather64_global(ef64, i + sj * j + sk * k, sj, sk, di, dj, dk);
+        } else {
+            float3 e = gather_global32(g, ef32, i, j, k, (float)di, (float)dj, (float)dk);
+            E = make_double3(e.x, e.y, e.z);
+        }
+        vx -= (E.x * qm) * half_dt;
+        vy -= (E.y * qm) * half_dt;
+        vz -= (E.z * qm) * half_dt;
+    }
+    long idx = dst.cap_bins + stage0 + q;
+    if (PREC == PIC_MIXED) {
+        PMixed p;
+        p.ci = i;
+        p.cj = j;
+        p.ck = k;
+        float fo[3] = {(float)di, (float)dj, (float)dk};
+        int *cc[3] = {&p.ci, &p.cj, &p.ck};
+        int nc[3] = {g.nci, g.ncj, g.nck};
+        for (int a = 0; a < 3; a++) {
+            if (fo[a] >= 1.0f) {  // offset rounded up to one cell
+                if (*cc[a] + 1 < nc[a]) {
+                    fo[a] = 0.0f;
+                    *cc[a] += 1;
+                } else {
+                    fo[a] = 0x1.fffffep-1f;
+                }
+            }
+        }
+        p.ox = fo[0];
+        p.oy = fo[1];
+        p.oz = fo[2];
+        p.vx = (float)vx;
+        p.vy = (float)vy;
+        p.vz = (float)vz;
+        p.w = (float)mpw[q];
+        p.id = ids ? ids[q] : id0 + (uint32_t)q;
+        store_p(dst, idx, p);
+    } else {
+        PF64 p{x, y, z, vx, vy, vz, mpw[q], ids ? ids[q] : id0 + (uint32_t)q};
+        store_p(dst, idx, p);
+    }
+}
+
+// Gathers bins + staging into contiguous FP64 arrays (device order).
+template <int PREC>
+__global__ void __launch_bounds__(256)
+    k_gather_out(Grid g, PartView in, const uint32_t *offsets, double *pos, double *vel, double *mpw,
+                 uint32_t *ids) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    int b = blockIdx.x;
+    uint32_t cnt;
+    long base;
+    uint32_t step, first, o;
+    if (b < g.ntiles) {
+        cnt = clamped_count(in.bin_count, in.bin_cap, b);
+        base = in.bin_start[b];
+        first = threadIdx.x;
+        step = blockDim.x;
+        o = offsets[b];
+    } else {
+        cnt = min(*in.stage_count, (uint32_t)in.cap_stage);
+        base = in.cap_bins;
+        first = (b - g.ntiles) * blockDim.x + threadIdx.x;
+        step = (gridDim.x - g.ntiles) * blockDim.x;
+        o = offsets[g.ntiles];
+    }
+    for (uint32_t q = first; q < cnt; q += step) {
+        PT p;
+        load_p(in, base + q, p);
+        long d = (long)o + q;
+        if constexpr (PREC == PIC_MIXED) {
+            pos[3 * d + 0] = g.x0[0] + ((double)p.ci + (double)p.ox) * g.dh[0];
+            pos[3 * d + 1] = g.x0[1] + ((double)p.cj + (double)p.oy) * g.dh[1];
+            pos[3 * d + 2] = g.x0[2] + ((double)p.ck + (double)p.oz) * g.dh[2];
+        } else {
+            pos[3 * d + 0] = p.x;
+            pos[3 * d + 1] = p.y;
+            pos[3 * d + 2] = p.z;
+        }
+        vel[3 * d + 0] = p.vx;
+        vel[3 * d + 1] = p.vy;
+        vel[3 * d + 2] = p.vz;
+        mpw[d] = p.w;
+        if (ids) ids[d] = p.id;
+    }
+}
+
+// Device beam / box source.  Particle q of this call uses Philox counter (q, ctr_lo, ctr_hi,
+// stream) with key (seed); draws: 2 for the position on the face (or 3 in a box), then for a
+// warm beam the isotropic direction (2) and the speed (9), as ch4/Species.cpp:149-174.
+struct InjectParams {
+    Grid g;
+    long n;
+    int box;         // 0: k=0 face (beam source), 1: box [x1,x2)
+    double x1[3], x2[3];
+    double v_drift;
+    double vth;      // sqrt(2 k T / m) (0 for a cold beam)
+    double w;
+    double qm, half_dt;
+    int rewind;
+    uint32_t key0, key1, ctr_lo, ctr_hi;
+    uint32_t id0;
+};
+
+template <int PREC>
+__global__ void __launch_bounds__(256)
+    k_inject(InjectParams I, const float4 *ef32, const double4 *ef64, PartView dst, uint32_t stage0,
+             long flat0, uint32_t *err) {
+    long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (q >= I.n) return;
+    const Grid &g = I.g;
+    double u[16];
+    for (int blk = 0; blk < 4; blk++) {
+        U4 r = philox4x32_10(U4{(uint32_t)q, (uint32_t)(q >> 32) ^ I.ctr_lo, I.ctr_hi, (uint32_t)blk},
+                             I.key0, I.key1);
+        u[4 * blk + 0] = u01(r.x, r.y);
+        u[4 * blk + 1] = u01(r.z, r.w);
+        U4 r2 = philox4x32_10(U4{(uint32_t)q, (uint32_t)(q >> 32) ^ I.ctr_lo, I.ctr_hi, (uint32_t)blk + 4u},
+                              I.key0, I.key1);
+        u[4 * blk + 2] = u01(r2.x, r2.y);
+        u[4 * blk + 3] = u01(r2.z, r2.w);
+    }
+    double x, y, z;
+    if (I.box) {
+        x = I.x1[0] + u[0] * (I.x2[0] - I.x1[0]);
+        y = I.x1[1] + u[1] * (I.x2[1] - I.x1[1]);
+        z = I.x1[2] + u[2] * (I.x2[2] - I.x1[2]);
+    } else {
+        x = g.x0[0] + u[0] * (g.dh[0] * (g.ni - 1));
+        y = g.x0[1] + u[1] * (g.dh[1] * (g.nj - 1));
+        z = g.x0[2];
+    }
+    double vx = 0, vy = 0, vz = I.v_drift;
+    if (I.vth > 0) {
+        double theta = 2 * PI_REF * u[3];
+        double r = -1.0 + 2 * u[4];
+        double a = sqrt(1 - r * r);
+        double v1 = I.vth * (u[5] + u[6] + u[7] - 1.5);
+        double v2 = I.vth * (u[8] + u[9] + u[10] - 1.5);
+        double v3 = I.vth * (u[11] + u[12] + u[13] - 1.5);
+        double vmag = 3 / sqrt(2.0 + 2 + 2) * sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+        vx = r * vmag;
+        vy = cos(theta) * a * vmag;
+        vz += sin(theta) * a * vmag;
+    }
+    // clamp into [x0,xm): a box upper bound equal to xm may round onto it
+    double pp[3] = {x, y, z};
+    int cc[3];
+    double dd[3];
+    int nc[3] = {g.nci, g.ncj, g.nck};
+    for (int a = 0; a < 3; a++) {
+        double l = (pp[a] - g.x0[a]) / g.dh[a];
+        int c = (int)l;
+        if (c < 0) c = 0;
+        if (c >= nc[a]) c = nc[a] - 1;
+        cc[a] = c;
+        dd[a] = l - c;
+        if (dd[a] >= 1.0) dd[a] = 0.9999999999999999;
+    }
+    if (I.rewind) {
+        long sj = g.ni, sk = (long)g.ni * g.nj;
+        double3 E;
+        if (PREC == PIC_FP64) {
+            E = gather64_global(ef64, cc[0] + sj * cc[1] + sk * cc[2], sj, sk, dd[0], dd[1], dd[2]);
+        } else {
+            float3 e = gather_global32(g, ef32, cc[0], cc[1], cc[2], (float)dd[0], (float)dd[1], (float)dd[2]);
+            E = make_double3(e.x, e.y, e.z);
+        }
+        vx -= (E.x * I.qm) * I.half_dt;
+        vy -= (E.y * I.qm) * I.half_dt;
+        vz -= (E.z * I.qm) * I.half_dt;
+    }
+    long idx = flat0 >= 0 ? flat0 + q : dst.cap_bins + stage0 + q;
+    if (PREC == PIC_MIXED) {
+        PMixed p;
+        p.ci = cc[0];
+        p.cj = cc[1];
+        p.ck = cc[2];
+        float fo[3];
+        for (int a = 0; a < 3; a++) {
+            fo[a] = (float)dd[a];
+            if (fo[a] >= 1.0f) fo[a] = 0x1.fffffep-1f;
+        }
+        p.ox = fo[0];
+        p.oy = fo[1];
+        p.oz = fo[2];
+        p.vx = (float)vx;
+        p.vy = (float)vy;
+        p.vz = (float)vz;
+        p.w = (float)I.w;
+        p.id = I.id0 + (uint32_t)q;
+        store_p(dst, idx, p);
+    } else {
+        // keep the sampled position; one that rounded onto an upper wall moves just inside
+        PF64 p{x, y, z, vx, vy, vz, I.w, I.id0 + (uint32_t)q};
+        if (p.x >= g.xm[0]) p.x = nextafter(g.xm[0], g.x0[0]);
+        if (p.y >= g.xm[1]) p.y = nextafter(g.xm[1], g.x0[1]);
+        if (p.z >= g.xm[2]) p.z = nextafter(g.xm[2], g.x0[2]);
+        store_p(dst, idx, p);
+    }
+}
+
+__global__ void k_avg(long nn, double *ave, const double *inst, int samples) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u >= nn) return;
+    // Field_::updateAverage (ch3/ver2/Field.h:214-220)
+    ave[u] = (inst[u] + samples * ave[u]) / (samples + 1);
+}
+
+__global__ void k_fill_u8(long n, uint8_t *p, uint8_t v) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u < n) p[u] = v;
+}
+
+__global__ void k_fill_f64(long n, double *p, double v) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u < n) p[u] = v;
+}
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+
+inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
+
+// =========================================================================================
+// host side
+// =========================================================================================
+static size_t part_bytes(int prec, bool ids) { return (prec == PIC_MIXED ? 32 : 56) + (ids ? 4 : 0); }
+
+static int alloc_buf(pic_species *s, PartBuf &b, long cap_bins, long cap_stage) {
+    const Grid &g = s->w->g;
+    long n = cap_bins + cap_stage;
+    size_t esz = s->prec == PIC_MIXED ? 4 : 8;
+    size_t bytes = 0;
+    size_t arr = (size_t)n * esz;
+    arr = (arr + 255) & ~(size_t)255;
+    bytes += arr * 7;
+    if (s->prec == PIC_MIXED) bytes += arr;  // cell array (4 B, same stride as the floats)
+    size_t idb = s->track_ids ? (((size_t)n * 4 + 255) & ~(size_t)255) : 0;
+    bytes += idb;
+    size_t binb = (((size_t)g.ntiles * 4 + 255) & ~(size_t)255);
+    bytes += binb * 3 + 256;
+    void *mem = nullptr;
+    PIC_HIP(hipMalloc(&mem, bytes));
+    b = PartBuf{};
+    b.mem = mem;
+    b.cap_bins = cap_bins;
+    b.cap_stage = cap_stage;
+    char *p = (char *)mem;
+    auto take = [&](size_t sz) {
+        char *r = p;
+        p += sz;
+        return r;
+    };
+    if (s->prec == PIC_MIXED) {
+        b.cell = (int *)take(arr);
+        b.ox = (float *)take(arr);
+        b.oy = (float *)take(arr);
+        b.oz = (float *)take(arr);
+        b.vx = (float *)take(arr);
+        b.vy = (float *)take(arr);
+        b.vz = (float *)take(arr);
+        b.w = (float *)take(arr);
+    } else {
+        b.x = (double *)take(arr);
+        b.y = (double *)take(arr);
+        b.z = (double *)take(arr);
+        b.dvx = (double *)take(arr);
+        b.dvy = (double *)take(arr);
+        b.dvz = (double *)take(arr);
+        b.dw = (double *)take(arr);
+    }
+    if (s->track_ids) b.id = (uint32_t *)take(idb);
+    b.bin_start = (uint32_t *)take(binb);
+    b.bin_count = (uint32_t *)take(binb);
+    b.bin_cap = (uint32_t *)take(binb);
+    b.stage_count = (uint32_t *)take(256);
+    // empty bins, empty staging
+    PIC_HIP(hipMemsetAsync(b.bin_start, 0, binb * 3 + 256, s->w->stream));
+    return PIC_OK;
+}
+
+static void free_buf(PartBuf &b) {
+    if (b.mem) hipFree(b.mem);
+    b = PartBuf{};
+}
+
+static long default_stage_cap(long cap_bins) { return std::max<long>(65536, cap_bins / 8); }
+
+static int launch_plan_push(pic_species *s, uint32_t *err) {
+    pic_world *w = s->w;
+    PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
+    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(w->g, in.bin_count, in.bin_cap, nullptr, 0, out.bin_start,
+                                              out.bin_cap, out.bin_count, out.stage_count, out.cap_bins,
+                                              s->np_dev, s->diag_dev, err);
+    PIC_LAUNCH_CHECK();
+    return PIC_OK;
+}
+
+// error flags of the last device operations live in DevScalars::counter slot of the world
+static uint32_t *err_flag(pic_species *s) { return &s->w->scal->counter; }
+
+static int check_err(pic_species *s) {
+    uint32_t e = 0;
+    PIC_HIP(hipMemcpyAsync(&s->w->scal_host->counter, err_flag(s), 4, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipStreamSynchronize(s->w->stream));
+    e = s->w->scal_host->counter;
+    if (e) {
+        k_set_u32<<<1, 1, 0, s->w->stream>>>(err_flag(s), 0u);
+        set_error(e & 2 ? "particle staging list overflowed (capacity)"
+                        : "particle buffers too small for the particle count (capacity)");
+        return PIC_ERR_CAPACITY;
+    }
+    return PIC_OK;
+}
+
+static int do_sort(pic_species *s);
+
+// grows both buffers so that cap_bins >= need (keeps the particles)
+static int ensure_capacity(pic_species *s, long need_bins, long need_stage) {
+    PartBuf &cur = s->buf[s->cur];
+    const Grid &g = s->w->g;
+    long want_bins = cur.cap_bins, want_stage = cur.cap_stage;
+    long slack_min = (long)g.ntiles * 48;
+    if (need_bins + need_bins / 4 + slack_min > want_bins) want_bins = need_bins + need_bins / 2 + slack_min;
+    if (need_stage > want_stage) want_stage = std::max(need_stage + need_stage / 2, default_stage_cap(want_bins));
+    if (want_bins == cur.cap_bins && want_stage == cur.cap_stage) return PIC_OK;
+    PIC_CHECK(want_bins + want_stage < 4000000000L, PIC_ERR_CAPACITY, "more than 4e9 particle slots");
+    // new buffers: sort from the current layout into the new one
+    PartBuf nb, nb2;
+    int rc = alloc_buf(s, nb, want_bins, want_stage);
+    if (rc) return rc;
+    rc = alloc_buf(s, nb2, want_bins, want_stage);
+    if (rc) {
+        free_buf(nb);
+        return rc;
+    }
+    PartBuf old_other = s->buf[1 - s->cur];
+    s->buf[1 - s->cur] = nb;  // sort target
+    rc = do_sort(s);          // moves cur -> nb and flips cur
+    // old current buffer is now the "other": replace it
+    PartBuf old_cur = s->buf[1 - s->cur];
+    s->buf[1 - s->cur] = nb2;
+    PIC_HIP(hipStreamSynchronize(s->w->stream));
+    free_buf(old_cur);
+    free_buf(old_other);
+    return rc;
+}
+
+template <int PREC>
+static int sort_launch(pic_species *s) {
+    pic_world *w = s->w;
+    const Grid &g = w->g;
+    PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
+    PartView vin = view_of(in), vout = view_of(out);
+    hipEvent_t ev;
+    timer_begin(w, PIC_T_SORT, &ev);
+    // histogram into out.bin_count (zeroed), then plan caps/starts from it
+    PIC_HIP(hipMemsetAsync(out.bin_count, 0, sizeof(uint32_t) * g.ntiles, w->stream));
+    unsigned grid = g.ntiles + STAGE_WGS * 4;
+    k_sort_hist<PREC><<<grid, 256, 0, w->stream>>>(g, vin, out.bin_count);
+    PIC_LAUNCH_CHECK();
+    // hist must survive the plan, which zeroes the counts: copy it to the cap array first
+    PIC_HIP(hipMemcpyAsync(out.bin_cap, out.bin_count, sizeof(uint32_t) * g.ntiles,
+                           hipMemcpyDeviceToDevice, w->stream));
+    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(g, nullptr, nullptr, out.bin_cap, 1, out.bin_start,
+                                              out.bin_cap, out.bin_count, out.stage_count, out.cap_bins,
+                                              nullptr, nullptr, err_flag(s));
+    PIC_LAUNCH_CHECK();
+    k_sort_scatter<PREC><<<grid, 256, 0, w->stream>>>(g, vin, vout, err_flag(s));
+    PIC_LAUNCH_CHECK();
+    timer_end(w, PIC_T_SORT, ev);
+    s->cur = 1 - s->cur;
+    s->sorted = true;
+    s->stage_added = 0;
+    return PIC_OK;
+}
+
+static int do_sort(pic_species *s) {
+    return s->prec == PIC_MIXED ? sort_launch<PIC_MIXED>(s) : sort_launch<PIC_FP64>(s);
+}
+
+// host-side count of particles in the current buffer (bins + staging); syncs
+static int count_particles(pic_species *s, std::vector<uint32_t> &cnt, uint32_t &stage) {
+    const Grid &g = s->w->g;
+    PartBuf &b = s->buf[s->cur];
+    std::vector<uint32_t> cap(g.ntiles);
+    cnt.resize(g.ntiles);
+    PIC_HIP(hipMemcpyAsync(cnt.data(), b.bin_count, 4 * (size_t)g.ntiles, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipMemcpyAsync(cap.data(), b.bin_cap, 4 * (size_t)g.ntiles, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipMemcpyAsync(&s->w->scal_host->iters, b.stage_count, 4, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipStreamSynchronize(s->w->stream));
+    for (int t = 0; t < g.ntiles; t++) cnt[t] = std::min(cnt[t], cap[t]);
+    stage = std::min<uint32_t>((uint32_t)s->w->scal_host->iters, (uint32_t)b.cap_stage);
+    return PIC_OK;
+}
+
+}  // namespace pic
+
+using namespace pic;
+
+extern "C" {
+
+int pic_species_create(pic_world *w, double mass, double charge, double mpw0, int precision,
+                       long capacity, int track_ids, pic_species **out) {
+    PIC_CHECK(w && out, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(precision == PIC_MIXED || precision == PIC_FP64, PIC_ERR_ARG, "bad precision");
+    PIC_CHECK(mass > 0, PIC_ERR_ARG, "mass must be positive");
+    PIC_CHECK(w->extents_set, PIC_ERR_STATE, "setExtents must precede species creation");
+    *out = nullptr;
+    pic_species *s = new pic_species();
+    s->w = w;
+    s->mass = mass;
+    s->charge = charge;
+    s->mpw0 = mpw0;
+    s->prec = precision;
+    s->track_ids = track_ids != 0;
+    const Grid &g = w->g;
+    long cap = std::max<long>(capacity, 1024);
+    long cap_bins = cap + cap / 2 + (long)g.ntiles * 48;
+    long cap_stage = default_stage_cap(cap_bins);
+    int rc = PIC_OK;
+    size_t wbytes = (size_t)g.ntiles * WNODES * (precision == PIC_MIXED ? 4 : 8);
+    if ((rc = alloc_buf(s, s->buf[0], cap_bins, cap_stage))) goto fail;
+    if ((rc = alloc_buf(s, s->buf[1], cap_bins, cap_stage))) goto fail;
+#define TRY(x)                                                 \
+    do {                                                       \
+        hipError_t e_ = (x);                                   \
+        if (e_ != hipSuccess) {                                \
+            rc = hip_fail(e_, #x, __FILE__, __LINE__);         \
+            goto fail;                                         \
+        }                                                      \
+    } while (0)
+    TRY(hipMalloc(&s->np_dev, sizeof(long)));
+    TRY(hipHostMalloc(&s->np_host, sizeof(long)));
+    TRY(hipMalloc(&s->den, sizeof(double) * g.nn));
+    TRY(hipMalloc(&s->den_extra, sizeof(double) * g.nn));
+    TRY(hipMalloc(&s->den_ave, sizeof(double) * g.nn));
+    TRY(hipMalloc(&s->win, wbytes));
+    TRY(hipMalloc(&s->win_used, g.ntiles));
+    TRY(hipMalloc(&s->diag_dev, sizeof(double) * 8));
+    TRY(hipHostMalloc(&s->diag_host, sizeof(double) * 8));
+    TRY(hipMemsetAsync(s->np_dev, 0, sizeof(long), w->stream));
+    TRY(hipMemsetAsync(s->den, 0, sizeof(double) * g.nn, w->stream));
+    TRY(hipMemsetAsync(s->den_extra, 0, sizeof(double) * g.nn, w->stream));
+    TRY(hipMemsetAsync(s->den_ave, 0, sizeof(double) * g.nn, w->stream));
+    TRY(hipMemsetAsync(s->win_used, 0, g.ntiles, w->stream));
+    TRY(hipMemsetAsync(s->diag_dev, 0, sizeof(double) * 8, w->stream));
+    TRY(hipStreamSynchronize(w->stream));
+#undef TRY
+    *s->np_host = 0;
+    s->den_state = 2;  // no particles: den = 0 is current
+    s->sorted = true;
+    *out = s;
+    return PIC_OK;
+fail:
+    pic_species_destroy(s);
+    return rc;
+}
+
+int pic_species_destroy(pic_species *s) {
+    if (!s) return PIC_OK;
+    if (s->w && s->w->stream) hipStreamSynchronize(s->w->stream);
+    free_buf(s->buf[0]);
+    free_buf(s->buf[1]);
+    hipFree(s->np_dev);
+    if (s->np_host) hipHostFree(s->np_host);
+    hipFree(s->den);
+    hipFree(s->den_extra);
+    hipFree(s->den_ave);
+    hipFree(s->win);
+    hipFree(s->win_used);
+    hipFree(s->diag_dev);
+    if (s->diag_host) hipHostFree(s->diag_host);
+    delete s;
+    return PIC_OK;
+}
+
+static int append_host(pic_species *s, long n, const double *pos, const double *vel, const double *mpw,
+                       const uint32_t *ids, int rewind, double dt) {
+    pic_world *w = s->w;
+    if (n == 0) return PIC_OK;
+    // staging must hold the new particles on top of what it already holds
+    std::vector<uint32_t> cnt;
+    uint32_t stage = 0;
+    int rc = count_particles(s, cnt, stage);
+    if (rc) return rc;
+    long inbins = 0;
+    for (auto c : cnt) inbins += c;
+    rc = ensure_capacity(s, inbins + stage + n, stage + n);
+    if (rc) return rc;
+    rc = count_particles(s, cnt, stage);  // a regrow sorts staging into the bins
+    if (rc) return rc;
+    if (s->buf[s->cur].cap_stage < stage + n) {
+        // staging still short after growth: sort what is there into the bins first
+        rc = do_sort(s);
+        if (rc) return rc;
+        stage = 0;
+    }
+    PartBuf &b = s->buf[s->cur];
+    double *d = nullptr;
+    size_t bytes = sizeof(double) * (size_t)n * 7 + (ids ? 4 * (size_t)n : 0);
+    PIC_HIP(hipMalloc(&d, bytes));
+    double *dpos = d, *dvel = d + 3 * n, *dmpw = d + 6 * n;
+    uint32_t *dids = ids ? (uint32_t *)(d + 7 * n) : nullptr;
+    PIC_HIP(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w->stream));
+    PIC_HIP(hipMemcpyAsync(dvel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w->stream));
+    PIC_HIP(hipMemcpyAsync(dmpw, mpw, sizeof(double) * n, hipMemcpyHostToDevice, w->stream));
+    if (ids) PIC_HIP(hipMemcpyAsync(dids, ids, 4 * (size_t)n, hipMemcpyHostToDevice, w->stream));
+    double qm = s->charge / s->mass;
+    if (s->prec == PIC_MIXED)
+        k_append<PIC_MIXED><<<nb256(n), 256, 0, w->stream>>>(w->g, n, dpos, dvel, dmpw, dids, s->next_id, rewind,
+                                                             qm, 0.5 * dt, w->ef32, w->ef64, view_of(b), stage);
+    else
+        k_append<PIC_FP64><<<nb256(n), 256, 0, w->stream>>>(w->g, n, dpos, dvel, dmpw, dids, s->next_id, rewind,
+                                                            qm, 0.5 * dt, w->ef32, w->ef64, view_of(b), stage);
+    PIC_LAUNCH_CHECK();
+    k_set_u32<<<1, 1, 0, w->stream>>>(b.stage_count, stage + (uint32_t)n);
+    PIC_HIP(hipStreamSynchronize(w->stream));
+    PIC_HIP(hipFree(d));
+    if (!ids) s->next_id += (uint32_t)n;
+    s->np_upper = inbins + stage + n;
+    *s->np_host = s->np_upper;
+    s->stage_added += n;
+    s->sorted = false;
+    s->den_state = 0;
+    s->diag_valid = false;
+    return PIC_OK;
+}
+
+int pic_species_add_particles(pic_species *s, long n, const double *pos, const double *vel,
+                              const double *mpw, double dt, long *n_added) {
+    PIC_CHECK(s && (n == 0 || (pos && vel && mpw)), PIC_ERR_ARG, "null argument");
+    const Grid &g = s->w->g;
+    // World::inBounds (ch2/World.h:59-63) filter on the host, keeping call order
+    std::vector<double> P, V, M;
+    P.reserve(3 * n);
+    V.reserve(3 * n);
+    M.reserve(n);
+    for (long q = 0; q < n; q++) {
+        const double *x = pos + 3 * q;
+        bool in = true;
+        for (int c = 0; c < 3; c++)
+            if (x[c] < g.x0[c] || x[c] >= g.xm[c]) in = false;
+        if (!in) continue;
+        for (int c = 0; c < 3; c++) {
+            P.push_back(x[c]);
+            V.push_back(vel[3 * q + c]);
+        }
+        M.push_back(mpw[q]);
+    }
+    long k = (long)M.size();
+    if (n_added) *n_added = k;
+    return append_host(s, k, P.data(), V.data(), M.data(), nullptr, 1, dt);
+}
+
+int pic_species_set_particles(pic_species *s, long n, const double *pos, const double *vel,
+                              const double *mpw, const uint32_t *ids) {
+    PIC_CHECK(s && (n == 0 || (pos && vel && mpw)), PIC_ERR_ARG, "null argument");
+    pic_world *w = s->w;
+    // drop everything: empty bins and staging
+    for (int b = 0; b < 2; b++) {
+        PartBuf &pb = s->buf[b];
+        PIC_HIP(hipMemsetAsync(pb.bin_count, 0, 4 * (size_t)w->g.ntiles, w->stream));
+        PIC_HIP(hipMemsetAsync(pb.bin_cap, 0, 4 * (size_t)w->g.ntiles, w->stream));
+        PIC_HIP(hipMemsetAsync(pb.stage_count, 0, 4, w->stream));
+    }
+    s->next_id = 0;
+    s->np_upper = 0;
+    *s->np_host = 0;
+    if (n == 0) {
+        s->den_state = 0;
+        s->diag_valid = false;
+        return PIC_OK;
+    }
+    int rc = append_host(s, n, pos, vel, mpw, ids, 0, 0.0);
+    if (rc) return rc;
+    return do_sort(s);
+}
+
+int pic_species_get_np(pic_species *s, long *np) {
+    PIC_CHECK(s && np, PIC_ERR_ARG, "null argument");
+    std::vector<uint32_t> cnt;
+    uint32_t stage = 0;
+    int rc = count_particles(s, cnt, stage);
+    if (rc) return rc;
+    long tot = stage;
+    for (auto c : cnt) tot += c;
+    *np = tot;
+    s->np_upper = tot;
+    return check_err(s);
+}
+
+int pic_species_get_particles(pic_species *s, double *pos, double *vel, double *mpw, uint32_t *ids) {
+    PIC_CHECK(s && pos && vel && mpw, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(!ids || s->track_ids, PIC_ERR_ARG, "species does not track ids");
+    pic_world *w = s->w;
+    const Grid &g = w->g;
+    std::vector<uint32_t> cnt;
+    uint32_t stage = 0;
+    int rc = count_particles(s, cnt, stage);
+    if (rc) return rc;
+    std::vector<uint32_t> off(g.ntiles + 1);
+    long run = 0;
+    for (int t = 0; t < g.ntiles; t++) {
+        off[t] = (uint32_t)run;
+        run += cnt[t];
+    }
+    off[g.ntiles] = (uint32_t)run;
+    long n = run + stage;
+    if (n == 0) return PIC_OK;
+    uint32_t *doff = nullptr;
+    double *d = nullptr;
+    PIC_HIP(hipMalloc(&doff, 4 * (size_t)(g.ntiles + 1)));
+    PIC_HIP(hipMalloc(&d, sizeof(double) * 7 * (size_t)n + 4 * (size_t)n));
+    PIC_HIP(hipMemcpyAsync(doff, off.data(), 4 * (size_t)(g.ntiles + 1), hipMemcpyHostToDevice, w->stream));
+    double *dpos = d, *dvel = d + 3 * n, *dmpw = d + 6 * n;
+    uint32_t *dids = ids ? (uint32_t *)(d + 7 * n) : nullptr;
+    unsigned grid = g.ntiles + STAGE_WGS;
+    PartView v = view_of(s->buf[s->cur]);
+    if (s->prec == PIC_MIXED)
+        k_gather_out<PIC_MIXED><<<grid, 256, 0, w->stream>>>(g, v, doff, dpos, dvel, dmpw, dids);
+    else
+        k_gather_out<PIC_FP64><<<grid, 256, 0, w->stream>>>(g, v, doff, dpos, dvel, dmpw, dids);
+    PIC_LAUNCH_CHECK();
+    PIC_HIP(hipMemcpyAsync(pos, dpos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipMemcpyAsync(vel, dvel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipMemcpyAsync(mpw, dmpw, sizeof(double) * n, hipMemcpyDeviceToHost, w->stream));
+    if (ids) PIC_HIP(hipMemcpyAsync(ids, dids, 4 * (size_t)n, hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipStreamSynchronize(w->stream));
+    PIC_HIP(hipFree(d));
+    PIC_HIP(hipFree(doff));
+    return PIC_OK;
+}
+
+int pic_species_sort(pic_species *s) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    int rc = do_sort(s);
+    if (rc) return rc;
+    return check_err(s);
+}
+
+int pic_species_advance(pic_species *s, int bc, double dt) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(bc == PIC_BC_REFLECT || bc == PIC_BC_ABSORB, PIC_ERR_ARG, "bad boundary");
+    pic_world *w = s->w;
+    const Grid &g = w->g;
+    int rc;
+    // the push uses whatever E the world holds (computeEF or a host write), as the
+    // reference's advance() reads world.ef as it stands
+    // room for one more step: output bins need the current population plus slack
+    long up = s->np_upper;
+    PartBuf &in0 = s->buf[s->cur];
+    if (up + up / 4 + (long)g.ntiles * 48 > in0.cap_bins) {
+        rc = ensure_capacity(s, up, 0);
+        if (rc) return rc;
+    }
+    // a large unbinned list goes through the sort rather than the slow global path
+    if (!s->sorted && s->stage_added > 65536 && s->stage_added * 16 > up) {
+        rc = do_sort(s);
+        if (rc) return rc;
+    }
+    PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
+    rc = launch_plan_push(s, err_flag(s));
+    if (rc) return rc;
+    PushParams P{};
+    P.g = g;
+    P.bc = bc;
+    P.sphere_on = (bc == PIC_BC_ABSORB) && w->sphere.on;
+    P.remap = (g.ntiles % 8 == 0 && g.ntiles >= 64) ? 1 : 0;
+    P.qm_dt = (float)(dt * s->charge / s->mass);
+    P.s64 = dt * s->charge / s->mass;
+    P.dt = dt;
+    for (int c = 0; c < 3; c++) {
+        P.dt_dh[c] = (float)(dt / g.dh[c]);
+        P.dhf[c] = (float)g.dh[c];
+        double Ls = (w->sphere.c[c] - g.x0[c]) / g.dh[c];
+        double fl = floor(Ls);
+        P.sph_c[c] = (int)fl;
+        P.sph_f[c] = (float)(Ls - fl);
+        P.sph_c64[c] = w->sphere.c[c];
+    }
+    P.r2f = (float)w->sphere.r2;
+    P.sph_r2 = w->sphere.r2;
+    PushOut O{};
+    O.win32 = (float *)s->win;
+    O.win64 = (double *)s->win;
+    O.win_used = s->win_used;
+    O.den_extra = s->den_extra;
+    O.np_dev = s->np_dev;
+    O.diag = s->diag_dev;
+    O.err = err_flag(s);
+    unsigned grid = g.ntiles + STAGE_WGS;
+    hipEvent_t ev;
+    timer_begin(w, PIC_T_PUSH, &ev);
+    if (s->prec == PIC_MIXED)
+        k_push<PIC_MIXED><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+    else
+        k_push<PIC_FP64><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+    PIC_LAUNCH_CHECK();
+    timer_end(w, PIC_T_PUSH, ev);
+    s->cur = 1 - s->cur;
+    s->den_state = 1;
+    s->diag_valid = true;
+    s->sorted = true;  // overflow, if any, is in staging and gets pushed next step
+    s->stage_added = 0;
+    return PIC_OK;
+}
+
+int pic_species_compute_number_density(pic_species *s) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    pic_world *w = s->w;
+    const Grid &g = w->g;
+    if (s->den_state == 2) return PIC_OK;
+    hipEvent_t ev;
+    timer_begin(w, PIC_T_DEPOSIT, &ev);
+    if (s->den_state == 0) {
+        k_fill_u8<<<nb256(g.ntiles), 256, 0, w->stream>>>(g.ntiles, s->win_used, 0);
+        unsigned grid = g.ntiles + STAGE_WGS;
+        PartView v = view_of(s->buf[s->cur]);
+        if (s->prec == PIC_MIXED)
+            k_deposit_all<PIC_MIXED><<<grid, 256, 0, w->stream>>>(g, v, s->den_extra);
+        else
+            k_deposit_all<PIC_FP64><<<grid, 256, 0, w->stream>>>(g, v, s->den_extra);
+        PIC_LAUNCH_CHECK();
+    }
+    if (s->prec == PIC_MIXED)
+        k_finish_density<float><<<nb256(g.nn), 256, 0, w->stream>>>(g, (const float *)s->win, s->win_used,
+                                                                    s->den_extra, s->den);
+    else
+        k_finish_density<double><<<nb256(g.nn), 256, 0, w->stream>>>(g, (const double *)s->win, s->win_used,
+                                                                     s->den_extra, s->den);
+    PIC_LAUNCH_CHECK();
+    timer_end(w, PIC_T_DEPOSIT, ev);
+    s->den_state = 2;
+    return PIC_OK;
+}
+
+int pic_species_get_den(pic_species *s, double *out) {
+    PIC_CHECK(s && out, PIC_ERR_ARG, "null argument");
+    PIC_HIP(hipMemcpyAsync(out, s->den, sizeof(double) * s->w->g.nn, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipStreamSynchronize(s->w->stream));
+    return PIC_OK;
+}
+
+int pic_species_get_den_ave(pic_species *s, double *out) {
+    PIC_CHECK(s && out, PIC_ERR_ARG, "null argument");
+    PIC_HIP(hipMemcpyAsync(out, s->den_ave, sizeof(double) * s->w->g.nn, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipStreamSynchronize(s->w->stream));
+    return PIC_OK;
+}
+
+int pic_species_update_averages(pic_species *s) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    k_avg<<<nb256(s->w->g.nn), 256, 0, s->w->stream>>>(s->w->g.nn, s->den_ave, s->den, s->ave_samples);
+    PIC_LAUNCH_CHECK();
+    s->ave_samples++;
+    return PIC_OK;
+}
+
+int pic_species_diag(pic_species *s, double out[5]) {
+    PIC_CHECK(s && out, PIC_ERR_ARG, "null argument");
+    pic_world *w = s->w;
+    if (!s->diag_valid) {
+        k_fill_f64<<<1, 64, 0, w->stream>>>(8, s->diag_dev, 0.0);
+        PIC_HIP(hipMemsetAsync(s->np_dev, 0, sizeof(long), w->stream));
+        unsigned grid = w->g.ntiles + STAGE_WGS;
+        PartView v = view_of(s->buf[s->cur]);
+        if (s->prec == PIC_MIXED)
+            k_diag_all<PIC_MIXED><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_dev, s->np_dev);
+        else
+            k_diag_all<PIC_FP64><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_dev, s->np_dev);
+        PIC_LAUNCH_CHECK();
+        s->diag_valid = true;
+    }
+    PIC_HIP(hipMemcpyAsync(s->diag_host, s->diag_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipStreamSynchronize(w->stream));
+    // getRealCount, getMomentum = mass*sum(v*mpw), getKE = 0.5*mass*sum(mpw*v^2)
+    out[0] = s->diag_host[0];
+    out[1] = s->diag_host[1] * s->mass;
+    out[2] = s->diag_host[2] * s->mass;
+    out[3] = s->diag_host[3] * s->mass;
+    out[4] = 0.5 * s->mass * s->diag_host[4];
+    return PIC_OK;
+}
+
+static int inject_common(pic_species *s, InjectParams &I, bool flat_ok) {
+    (void)flat_ok;
+    pic_world *w = s->w;
+    if (I.n <= 0) return PIC_OK;
+    std::vector<uint32_t> cnt;
+    uint32_t stage = 0;
+    // the host knows an upper bound of the population; read the exact staging count
+    PIC_HIP(hipMemcpyAsync(&w->scal_host->iters, s->buf[s->cur].stage_count, 4, hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipStreamSynchronize(w->stream));
+    stage = (uint32_t)w->scal_host->iters;
+    long n = I.n;
+    long need_total = s->np_upper + n;
+    int rc = PIC_OK;
+    if (stage + n > s->buf[s->cur].cap_stage || need_total + need_total / 4 > s->buf[s->cur].cap_bins) {
+        rc = ensure_capacity(s, need_total, stage + n);
+        if (rc) return rc;
+        PIC_HIP(hipMemcpyAsync(&w->scal_host->iters, s->buf[s->cur].stage_count, 4, hipMemcpyDeviceToHost,
+                               w->stream));
+        PIC_HIP(hipStreamSynchronize(w->stream));
+        stage = (uint32_t)w->scal_host->iters;
+        if (stage + n > s->buf[s->cur].cap_stage) {
+            rc = do_sort(s);
+            if (rc) return rc;
+            stage = 0;
+        }
+    }
+    PartBuf &b = s->buf[s->cur];
+    I.id0 = s->next_id;
+    hipEvent_t ev;
+    timer_begin(w, PIC_T_INJECT, &ev);
+    if (s->prec == PIC_MIXED)
+        k_inject<PIC_MIXED><<<nb256(n), 256, 0, w->stream>>>(I, w->ef32, w->ef64, view_of(b), stage, -1, err_flag(s));
+    else
+        k_inject<PIC_FP64><<<nb256(n), 256, 0, w->stream>>>(I, w->ef32, w->ef64, view_of(b), stage, -1, err_flag(s));
+    PIC_LAUNCH_CHECK();
+    k_set_u32<<<1, 1, 0, w->stream>>>(b.stage_count, stage + (uint32_t)n);
+    timer_end(w, PIC_T_INJECT, ev);
+    s->next_id += (uint32_t)n;
+    s->np_upper += n;
+    s->stage_added += n;
+    s->sorted = false;
+    s->den_state = 0;
+    s->diag_valid = false;
+    return PIC_OK;
+}
+
+int pic_species_inject_beam(pic_species *s, double v_drift, double den, double T, double dt, uint64_t seed,
+                            uint64_t counter, long *n_added) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(s->mpw0 > 0, PIC_ERR_ARG, "species needs mpw0 > 0 for injection");
+    pic_world *w = s->w;
+    const Grid &g = w->g;
+    // ColdBeamSource::sample (ch3/ver2/Source.cpp:6-18)
+    double Lx = g.dh[0] * (g.ni - 1);
+    double Ly = g.dh[1] * (g.nj - 1);
+    double A = Lx * Ly;
+    double num_real = den * v_drift * A * dt;
+    U4 r = philox4x32_10(U4{0xffffffffu, (uint32_t)counter, (uint32_t)(counter >> 32), 0xfffffffeu},
+                         (uint32_t)seed, (uint32_t)(seed >> 32));
+    int num_sim = (int)(num_real / s->mpw0 + u01(r.x, r.y));
+    if (n_added) *n_added = num_sim;
+    InjectParams I{};
+    I.g = g;
+    I.n = num_sim;
+    I.box = 0;
+    I.v_drift = v_drift;
+    I.vth = T > 0 ? sqrt(2 * KB * T / s->mass) : 0.0;
+    I.w = s->mpw0;
+    I.qm = s->charge / s->mass;
+    I.half_dt = 0.5 * dt;
+    I.rewind = 1;
+    I.key0 = (uint32_t)seed;
+    I.key1 = (uint32_t)(seed >> 32);
+    I.ctr_lo = (uint32_t)counter;
+    I.ctr_hi = (uint32_t)(counter >> 32);
+    return inject_common(s, I, false);
+}
+
+int pic_species_load_box_device(pic_species *s, const double x1[3], const double x2[3], double num_den,
+                                long num_mp, double v_drift, double T, uint64_t seed) {
+    PIC_CHECK(s && x1 && x2 && num_mp >= 0, PIC_ERR_ARG, "bad argument");
+    pic_world *w = s->w;
+    const Grid &g = w->g;
+    for (int c = 0; c < 3; c++)
+        PIC_CHECK(x1[c] >= g.x0[c] && x2[c] <= g.xm[c] && x2[c] > x1[c], PIC_ERR_ARG, "box outside the domain");
+    double box_vol = (x2[0] - x1[0]) * (x2[1] - x1[1]) * (x2[2] - x1[2]);
+    double mpw = num_den * box_vol / (double)num_mp;
+    // batches no larger than the staging list; each batch is sorted into the bins
+    int rc = ensure_capacity(s, s->np_upper + num_mp, 0);
+    if (rc) return rc;
+    long done = 0;
+    uint64_t counter = 0;
+    while (done < num_mp) {
+        long bs = std::min<long>(num_mp - done, s->buf[s->cur].cap_stage / 2);
+        InjectParams I{};
+        I.g = g;
+        I.n = bs;
+        I.box = 1;
+        for (int c = 0; c < 3; c++) {
+            I.x1[c] = x1[c];
+            I.x2[c] = x2[c];
+        }
+        I.v_drift = v_drift;
+        I.vth = T > 0 ? sqrt(2 * KB * T / s->mass) : 0.0;
+        I.w = mpw;
+        I.qm = s->charge / s->mass;
+        I.rewind = 0;
+        I.key0 = (uint32_t)seed;
+        I.key1 = (uint32_t)(seed >> 32);
+        I.ctr_lo = (uint32_t)counter;
+        I.ctr_hi = 0x5eed0000u;
+        // counter advances by the batch so that draws never repeat across batches
+        rc = inject_common(s, I, true);
+        if (rc) return rc;
+        rc = do_sort(s);
+        if (rc) return rc;
+        done += bs;
+        counter += 1;
+    }
+    return check_err(s);
+}
+
+}  // extern "C"

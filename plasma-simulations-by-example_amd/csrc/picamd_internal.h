@@ -1,0 +1,253 @@
+// Internal definitions of libpicamd (gfx950).  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "picamd.h"
+
+namespace pic {
+
+// ---- physical constants (ch2/World.h:11-20) -------------------------------------------
+constexpr double EPS_0 = 8.85418782e-12;
+constexpr double QE = 1.602176565e-19;
+constexpr double KB = 1.380648e-23;
+constexpr double PI_REF = 3.141592653;  // the reference's Const::PI
+
+// ---- tiling ---------------------------------------------------------------------------
+// Particles live in per-tile bins; a tile is TILE^3 cells.  The push of one tile is one
+// workgroup: its E tile (TILE+1)^3 nodes sits in LDS, and deposits of the pushed
+// particles land in an LDS window of the tile plus HALO cells on every side.
+constexpr int TILE_SHIFT = 3;
+constexpr int TILE = 1 << TILE_SHIFT;
+constexpr int TN = TILE + 1;              // E-tile nodes per edge
+constexpr int TNODES = TN * TN * TN;      // 729
+constexpr int HALO = 1;
+constexpr int WN = TILE + 1 + 2 * HALO;   // window nodes per edge
+constexpr int WNODES = WN * WN * WN;      // 1331
+constexpr int PUSH_THREADS = 256;
+constexpr int PUSH_PER_THREAD = 4;        // particles per thread per batch
+constexpr int PUSH_BATCH = PUSH_THREADS * PUSH_PER_THREAD;
+constexpr int NDEST = 27;                 // destination tiles reachable in one step (3x3x3)
+constexpr int STAGE_WGS = 64;             // workgroups that push the unbinned list
+
+// packed cell index of PIC_MIXED particles: i | j<<10 | k<<20
+constexpr int CELL_BITS_I = 10, CELL_BITS_J = 10, CELL_BITS_K = 11;
+constexpr int MAX_CELLS_I = 1 << CELL_BITS_I, MAX_CELLS_J = 1 << CELL_BITS_J,
+              MAX_CELLS_K = 1 << CELL_BITS_K;
+
+__host__ __device__ inline int pack_cell(int i, int j, int k) { return i | (j << 10) | (k << 20); }
+__host__ __device__ inline int cell_i(int c) { return c & 1023; }
+__host__ __device__ inline int cell_j(int c) { return (c >> 10) & 1023; }
+__host__ __device__ inline int cell_k(int c) { return (c >> 20) & 2047; }
+
+struct Grid {
+    int ni, nj, nk;     // nodes
+    int nci, ncj, nck;  // cells
+    int ntx, nty, ntz;  // tiles
+    int ntiles;
+    long nn;
+    double x0[3], xm[3], dh[3];
+};
+
+struct Sphere {
+    int on;
+    double c[3];
+    double r2;
+};
+
+// ---- particle storage ---------------------------------------------------------------
+// One buffer: [0, cap_bins) holds tile bins, [cap_bins, cap_bins+cap_stage) the unbinned
+// staging list (host uploads, injected particles, bin overflow).
+struct PartBuf {
+    void *mem = nullptr;
+    long cap_bins = 0, cap_stage = 0;
+    // PIC_MIXED
+    int *cell = nullptr;
+    float *ox = nullptr, *oy = nullptr, *oz = nullptr;
+    float *vx = nullptr, *vy = nullptr, *vz = nullptr, *w = nullptr;
+    // PIC_FP64
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *dvx = nullptr, *dvy = nullptr, *dvz = nullptr, *dw = nullptr;
+    uint32_t *id = nullptr;
+    // tile bins
+    uint32_t *bin_start = nullptr, *bin_count = nullptr, *bin_cap = nullptr;
+    uint32_t *stage_count = nullptr;  // device counter of the staging list
+};
+
+// Device view of a buffer, passed by value to kernels.
+struct PartView {
+    int *cell;
+    float *ox, *oy, *oz, *vx, *vy, *vz, *w;
+    double *x, *y, *z, *dvx, *dvy, *dvz, *dw;
+    uint32_t *id;
+    uint32_t *bin_start, *bin_count, *bin_cap, *stage_count;
+    long cap_bins, cap_stage;
+};
+
+inline PartView view_of(const PartBuf &b) {
+    return PartView{b.cell, b.ox, b.oy, b.oz, b.vx, b.vy, b.vz, b.w, b.x, b.y, b.z,
+                    b.dvx, b.dvy, b.dvz, b.dw, b.id, b.bin_start, b.bin_count, b.bin_cap,
+                    b.stage_count, b.cap_bins, b.cap_stage};
+}
+
+// ---- device scalars of a world ---------------------------------------------------------
+struct DevScalars {
+    double pe;            // getPE accumulator
+    double red[8];        // general reductions
+    unsigned int done;    // solver convergence flag
+    unsigned int counter; // last-block counter
+    int iters;
+    int pad;
+};
+
+struct TimerRec {
+    int phase;
+    hipEvent_t a, b;
+};
+
+}  // namespace pic
+
+struct pic_world {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pic::Grid g{};
+    bool extents_set = false;
+    double *phi = nullptr, *rho = nullptr, *node_vol = nullptr;
+    float4 *ef32 = nullptr;
+    double4 *ef64 = nullptr;
+    uint8_t *object_id = nullptr;
+    pic::Sphere sphere{};
+    pic::DevScalars *scal = nullptr;  // device
+    pic::DevScalars *scal_host = nullptr;  // pinned mirror
+    double *partials = nullptr;       // per-block reduction partials
+    int partials_cap = 0;
+    bool ef_valid = false;            // ef32/ef64 hold computeEF of the current phi
+    bool pe_valid = false;
+    // timers
+    bool timing = false;
+    std::vector<pic::TimerRec> pending;
+    std::vector<hipEvent_t> free_events;
+    double ms[PIC_T_COUNT] = {0};
+    long calls[PIC_T_COUNT] = {0};
+};
+
+struct pic_species {
+    pic_world *w = nullptr;
+    double mass = 0, charge = 0, mpw0 = 0;
+    int prec = PIC_MIXED;
+    bool track_ids = false;
+    pic::PartBuf buf[2];
+    int cur = 0;                  // buffer holding the current particles
+    long np_upper = 0;            // host-side upper bound of the particle count
+    uint32_t next_id = 0;
+    long *np_dev = nullptr;       // device particle counter written by the push
+    long *np_host = nullptr;      // pinned mirror
+    double *den = nullptr, *den_extra = nullptr, *den_ave = nullptr;
+    int ave_samples = 0;
+    void *win = nullptr;          // per-tile deposit windows (float or double)
+    uint8_t *win_used = nullptr;
+    double *diag_dev = nullptr;   // [5] fused diagnostics of the last push
+    double *diag_host = nullptr;  // pinned
+    bool diag_valid = false;      // diag_dev matches the current particles
+    int den_state = 0;            // 0 stale, 1 windows hold the last push deposit, 2 den final
+    bool sorted = false;          // particles are all in bins (staging empty)
+    long stage_added = 0;         // particles appended to staging since the last push or sort
+};
+
+struct pic_solver {
+    pic_world *w = nullptr;
+    int type = PIC_SOLVER_GS;
+    int max_it = 10000;
+    double tol = 1e-4;
+    double phi0 = 0, n0 = 1e12, Te0 = 1.5;  // defaults of ch3/ver2/PotentialSolver.h:93-95
+    // NR-PCG vectors
+    double *y = nullptr, *g = nullptr, *d = nullptr, *z = nullptr, *F = nullptr, *P = nullptr;
+};
+
+// ---- error handling ------------------------------------------------------------------
+namespace pic {
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+int grow_partials(pic_world *w, int nblocks);
+
+// phase timing helpers (no-ops unless timing is enabled)
+void timer_begin(pic_world *w, int phase, hipEvent_t *a);
+void timer_end(pic_world *w, int phase, hipEvent_t a);
+}  // namespace pic
+
+#define PIC_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) return pic::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define PIC_CHECK(cond, code, msg)       \
+    do {                                 \
+        if (!(cond)) {                   \
+            pic::set_error(msg);         \
+            return (code);               \
+        }                                \
+    } while (0)
+
+#define PIC_LAUNCH_CHECK() PIC_HIP(hipGetLastError())
+
+// ---- device helpers ------------------------------------------------------------------
+namespace pic {
+
+__device__ inline double warp_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block-wide FP64 sum; result valid in thread 0.  sh must hold >= blockDim/64 doubles.
+__device__ inline double block_sum(double v, double *sh) {
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = warp_sum(v);
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int nw = (blockDim.x + 63) >> 6;
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nw; i++) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11): counter-based, so every particle's draws depend only on (seed, counter).
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)M0 * ctr.x;
+        uint64_t p1 = (uint64_t)M1 * ctr.z;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        U4 n;
+        n.x = hi1 ^ ctr.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ ctr.w ^ k1;
+        n.w = lo0;
+        ctr = n;
+        k0 += W0;
+        k1 += W1;
+    }
+    return ctr;
+}
+
+// uniform double in [0,1) from two 32-bit words (53 random bits)
+__host__ __device__ inline double u01(uint32_t a, uint32_t b) {
+    uint64_t v = ((uint64_t)a << 21) ^ (uint64_t)(b >> 11);
+    return (double)(v & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace pic

@@ -1,0 +1,800 @@
+// PotentialSolver: linear SOR (ch2), nonlinear Boltzmann SOR (ch3 solveGS), Newton-PCG
+// (ch3 solveNRPCG + solvePCGLinear + solveGSLinear), solveQN and computeEF, on the device.
+//
+// Ordering.  The reference sweeps lexicographically (i outer, k inner).  On the GPU the
+// sweeps are red-black: same stencil, same relaxation factor, same residual definition and
+// stopping test, a different iterate path.  Converged solutions agree to the tolerance.
+// All arithmetic is FP64; dots and norms are FP64 sums with a fixed reduction tree.
+#include <algorithm>
+#include <cmath>
+
+#include "picamd_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace pic {
+
+int compute_ef(pic_world *w, bool want32, bool want64);
+int solve_qn(pic_world *w, double phi0, double Te0, double n0);
+
+struct SorParams {
+    double idx2, idy2, idz2;
+    double denom;  // 2*idx2 + 2*idy2 + 2*idz2
+    double w;      // relaxation factor
+    double phi0, Te0, n0;
+    double tol;
+    int max_it;
+};
+
+__device__ inline void node_ijk(const Grid &g, long u, int &i, int &j, int &k) {
+    i = (int)(u % g.ni);
+    long r = u / g.ni;
+    j = (int)(r % g.nj);
+    k = (int)(r / g.nj);
+}
+
+// ---- one node update -------------------------------------------------------------------
+// linear SOR, interior nodes only (ch2/PotentialSolver.cpp:32-42)
+__device__ inline double sor_lin_new(const double *phi, double rho, long u, long sj, long sk,
+                                     const SorParams &S) {
+    double phi_new = (rho / EPS_0 + S.idx2 * (phi[u - 1] + phi[u + 1]) + S.idy2 * (phi[u - sj] + phi[u + sj]) +
+                      S.idz2 * (phi[u - sk] + phi[u + sk])) /
+                     S.denom;
+    return phi[u] + S.w * (phi_new - phi[u]);
+}
+
+__device__ inline double sor_lin_res(const double *phi, double rho, long u, long sj, long sk,
+                                     const SorParams &S) {
+    return -phi[u] * S.denom + rho / EPS_0 + S.idx2 * (phi[u - 1] + phi[u + 1]) +
+           S.idy2 * (phi[u - sj] + phi[u + sj]) + S.idz2 * (phi[u - sk] + phi[u + sk]);
+}
+
+// nonlinear Boltzmann SOR (ch3/ver2/PotentialSolver.cpp:349-383); returns the new value
+__device__ inline double sor_boltz_new(const Grid &g, const double *phi, double rho, long u, int i, int j,
+                                       int k, const SorParams &S) {
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    if (i == 0) return phi[u + 1];
+    if (i == g.ni - 1) return phi[u - 1];
+    if (j == 0) return phi[u + sj];
+    if (j == g.nj - 1) return phi[u - sj];
+    if (k == 0) return phi[u + sk];
+    if (k == g.nk - 1) return phi[u - sk];
+    double ne = S.n0 * exp((phi[u] - S.phi0) / S.Te0);
+    double phi_new = ((rho - QE * ne) / EPS_0 + S.idx2 * (phi[u - 1] + phi[u + 1]) +
+                      S.idy2 * (phi[u - sj] + phi[u + sj]) + S.idz2 * (phi[u - sk] + phi[u + sk])) /
+                     S.denom;
+    return phi[u] + S.w * (phi_new - phi[u]);
+}
+
+// residual of solveGS (ch3/ver2/PotentialSolver.cpp:387-419)
+__device__ inline double sor_boltz_res(const Grid &g, const double *phi, double rho, long u, int i, int j,
+                                       int k, const SorParams &S) {
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    if (i == 0) return phi[u] - phi[u + 1];
+    if (i == g.ni - 1) return phi[u] - phi[u - 1];
+    if (j == 0) return phi[u] - phi[u + sj];
+    if (j == g.nj - 1) return phi[u] - phi[u - sj];
+    if (k == 0) return phi[u] - phi[u + sk];
+    if (k == g.nk - 1) return phi[u] - phi[u - sk];
+    double ne = S.n0 * exp((phi[u] - S.phi0) / S.Te0);
+    return -phi[u] * S.denom + (rho - QE * ne) / EPS_0 + S.idx2 * (phi[u - 1] + phi[u + 1]) +
+           S.idy2 * (phi[u - sj] + phi[u + sj]) + S.idz2 * (phi[u - sk] + phi[u + sk]);
+}
+
+// ---- single-workgroup solver with phi resident in LDS (small grids) -------------------
+// The whole iteration (sweeps, residual every 25 iterations, stopping test) runs in one
+// launch: no host round trip and no launch per sweep.
+constexpr int LDS_THREADS = 1024;
+
+template <bool BOLTZ>
+__global__ void __launch_bounds__(LDS_THREADS)
+    k_sor_lds(Grid g, double *phi, const double *__restrict__ rho, const uint8_t *__restrict__ oid,
+              SorParams S, DevScalars *out) {
+    extern __shared__ double sphi[];
+    __shared__ double sh[LDS_THREADS / 64];
+    __shared__ int s_conv;
+    long nn = g.nn;
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    for (long u = threadIdx.x; u < nn; u += LDS_THREADS) sphi[u] = phi[u];
+    if (threadIdx.x == 0) s_conv = 0;
+    __syncthreads();
+    double L2 = 0;
+    int it;
+    for (it = 0; it < S.max_it; it++) {
+        for (int color = 0; color < 2; color++) {
+            for (long u = threadIdx.x; u < nn; u += LDS_THREADS) {
+                int i, j, k;
+                node_ijk(g, u, i, j, k);
+                if (((i + j + k) & 1) != color) continue;
+                if (BOLTZ) {
+                    if (oid[u] > 0) continue;
+                    sphi[u] = sor_boltz_new(g, sphi, rho[u], u, i, j, k, S);
+                } else {
+                    if (i == 0 || j == 0 || k == 0 || i == g.ni - 1 || j == g.nj - 1 || k == g.nk - 1) continue;
+                    sphi[u] = sor_lin_new(sphi, rho[u], u, sj, sk, S);
+                }
+            }
+            __syncthreads();
+        }
+        if (it % 25 == 0) {
+            double sum = 0;
+            for (long u = threadIdx.x; u < nn; u += LDS_THREADS) {
+                int i, j, k;
+                node_ijk(g, u, i, j, k);
+                double R;
+                if (BOLTZ) {
+                    if (oid[u] > 0) continue;
+                    R = sor_boltz_res(g, sphi, rho[u], u, i, j, k, S);
+                } else {
+                    if (i == 0 || j == 0 || k == 0 || i == g.ni - 1 || j == g.nj - 1 || k == g.nk - 1) continue;
+                    R = sor_lin_res(sphi, rho[u], u, sj, sk, S);
+                }
+                sum += R * R;
+            }
+            double tot = block_sum(sum, sh);
+            if (threadIdx.x == 0) {
+                L2 = sqrt(tot / (double)(g.ni * g.nj * g.nk));
+                s_conv = L2 < S.tol;
+                sh[0] = L2;
+            }
+            __syncthreads();
+            L2 = sh[0];
+            int conv = s_conv;
+            __syncthreads();
+            if (conv) break;
+        }
+    }
+    for (long u = threadIdx.x; u < nn; u += LDS_THREADS) phi[u] = sphi[u];
+    if (threadIdx.x == 0) {
+        out->red[0] = L2;
+        out->done = s_conv;
+        out->iters = s_conv ? it + 1 : it;
+    }
+}
+
+// ---- multi-workgroup red-black sweeps (large grids) -------------------------------------
+template <bool BOLTZ>
+__global__ void __launch_bounds__(256)
+    k_sor_sweep(Grid g, double *phi, const double *__restrict__ rho, const uint8_t *__restrict__ oid,
+                SorParams S, int color, const DevScalars *sc) {
+    if (sc->done) return;
+    // thread -> pair of nodes along i; the one of this colour is updated
+    long half = (g.ni + 1) / 2;
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long rows = (long)g.nj * g.nk;
+    if (t >= half * rows) return;
+    long row = t / half;
+    int j = (int)(row % g.nj), k = (int)(row / g.nj);
+    int i = (int)(t % half) * 2;
+    if (((i + j + k) & 1) != color) i += 1;
+    if (i >= g.ni) return;
+    long u = i + g.ni * (long)j + (long)g.ni * g.nj * k;
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    if (BOLTZ) {
+        if (oid[u] > 0) return;
+        phi[u] = sor_boltz_new(g, phi, rho[u], u, i, j, k, S);
+    } else {
+        if (i == 0 || j == 0 || k == 0 || i == g.ni - 1 || j == g.nj - 1 || k == g.nk - 1) return;
+        phi[u] = sor_lin_new(phi, rho[u], u, sj, sk, S);
+    }
+}
+
+template <bool BOLTZ>
+__global__ void __launch_bounds__(256)
+    k_sor_residual(Grid g, const double *phi, const double *__restrict__ rho, const uint8_t *__restrict__ oid,
+                   SorParams S, double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double R2 = 0;
+    if (u < g.nn) {
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
+        long sj = g.ni, sk = (long)g.ni * g.nj;
+        if (BOLTZ) {
+            if (!(oid[u] > 0)) {
+                double R = sor_boltz_res(g, phi, rho[u], u, i, j, k, S);
+                R2 = R * R;
+            }
+        } else if (!(i == 0 || j == 0 || k == 0 || i == g.ni - 1 || j == g.nj - 1 || k == g.nk - 1)) {
+            double R = sor_lin_res(phi, rho[u], u, sj, sk, S);
+            R2 = R * R;
+        }
+    }
+    double s = block_sum(R2, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// one workgroup: sums nb partials per slot in a fixed order
+__device__ inline double fixed_sum(const double *p, int nb, double *sh) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += p[b];
+    return block_sum(s, sh);
+}
+
+__global__ void __launch_bounds__(1024)
+    k_sor_check(const double *partials, int nb, double denom_nodes, double tol, int it, DevScalars *sc) {
+    __shared__ double sh[16];
+    if (sc->done) return;
+    double tot = fixed_sum(partials, nb, sh);
+    if (threadIdx.x == 0) {
+        double L2 = sqrt(tot / denom_nodes);
+        sc->red[0] = L2;
+        if (L2 < tol) {
+            sc->done = 1;
+            sc->iters = it + 1;
+        }
+    }
+}
+
+// ---- Newton-PCG, matrix-free -----------------------------------------------------------
+// Row of A (buildMatrix, ch3/ver2/PotentialSolver.cpp:146-201) applied to v in the slot
+// order of the reference matrix; diag is the row's diagonal (A's, or J = A - diag(P)).
+enum { NT_REG = 0, NT_NEU = 1, NT_DIR = 2 };
+
+struct PcgParams {
+    Grid g;
+    double idx, idy, idz, idx2, idy2, idz2;
+    double phi0, Te0, n0;
+    double tol;
+};
+
+__device__ inline int node_type(const Grid &g, const uint8_t *oid, long u, int i, int j, int k) {
+    if (oid[u] > 0) return NT_DIR;
+    if (i == 0 || i == g.ni - 1 || j == 0 || j == g.nj - 1 || k == 0 || k == g.nk - 1) return NT_NEU;
+    return NT_REG;
+}
+
+// A's diagonal
+__device__ inline double a_diag(const PcgParams &Q, int type, int i, int j) {
+    if (type == NT_DIR) return 1.0;
+    if (type == NT_REG) return -2.0 * (Q.idx2 + Q.idy2 + Q.idz2);
+    const Grid &g = Q.g;
+    if (i == 0 || i == g.ni - 1) return Q.idx;
+    if (j == 0 || j == g.nj - 1) return Q.idy;
+    return Q.idz;
+}
+
+__device__ inline double row_apply(const PcgParams &Q, int type, long u, int i, int j, int k, double diag,
+                                   const double *v) {
+    const Grid &g = Q.g;
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    if (type == NT_DIR) return diag * v[u];
+    if (type == NT_REG) {
+        double s = 0;
+        s += Q.idz2 * v[u - sk];
+        s += Q.idy2 * v[u - sj];
+        s += Q.idx2 * v[u - 1];
+        s += diag * v[u];
+        s += Q.idx2 * v[u + 1];
+        s += Q.idy2 * v[u + sj];
+        s += Q.idz2 * v[u + sk];
+        return s;
+    }
+    double s = 0;
+    s += diag * v[u];
+    if (i == 0) s += -Q.idx * v[u + 1];
+    else if (i == g.ni - 1) s += -Q.idx * v[u - 1];
+    else if (j == 0) s += -Q.idy * v[u + sj];
+    else if (j == g.nj - 1) s += -Q.idy * v[u - sj];
+    else if (k == 0) s += -Q.idz * v[u + sk];
+    else s += -Q.idz * v[u - sk];
+    return s;
+}
+
+// F = A x - b, minus the Boltzmann term on regular nodes; P = d(ne term)/dphi
+// (ch3/ver2/PotentialSolver.cpp:240-264)
+__global__ void __launch_bounds__(256)
+    k_nr_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *P) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    if (u >= g.nn) return;
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
+    int t = node_type(g, oid, u, i, j, k);
+    double Ax = row_apply(Q, t, u, i, j, k, a_diag(Q, t, i, j), x);
+    double b;
+    if (t == NT_NEU) b = 0;
+    else if (t == NT_DIR) b = x[u];
+    else b = -rho[u] / EPS_0;
+    double f = Ax - b;
+    double p = 0;
+    if (t == NT_REG) {
+        f -= QE * Q.n0 * exp((x[u] - Q.phi0) / Q.Te0) / EPS_0;
+        p = Q.n0 * QE / (EPS_0 * Q.Te0) * exp((x[u] - Q.phi0) / Q.Te0);
+    }
+    F[u] = f;
+    P[u] = p;
+}
+
+__device__ inline double j_diag(const PcgParams &Q, int t, int i, int j, const double *P, long u) {
+    return a_diag(Q, t, i, j) - P[u];
+}
+
+// PCG init: g = J y - F, s = M g, d = -s; partials of g.s
+__global__ void __launch_bounds__(256)
+    k_pcg_init(PcgParams Q, const uint8_t *oid, const double *P, const double *y, const double *F, double *gv,
+               double *d, double *partials) {
+    __shared__ double sh[4];
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    double gs = 0;
+    if (u < g.nn) {
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
+        int t = node_type(g, oid, u, i, j, k);
+        double jd = j_diag(Q, t, i, j, P, u);
+        double gg = row_apply(Q, t, u, i, j, k, jd, y) - F[u];
+        double M = 1.0 / jd;
+        double s = M * gg;
+        gv[u] = gg;
+        d[u] = -1 * s;
+        gs = gg * s;
+    }
+    double r = block_sum(gs, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// z = J d; partials of d.z
+__global__ void __launch_bounds__(256)
+    k_pcg_spmv(PcgParams Q, const uint8_t *oid, const double *P, const double *d, double *z, double *partials,
+               const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    double dz = 0;
+    if (u < g.nn) {
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
+        int t = node_type(g, oid, u, i, j, k);
+        double zz = row_apply(Q, t, u, i, j, k, j_diag(Q, t, i, j, P, u), d);
+        z[u] = zz;
+        dz = d[u] * zz;
+    }
+    double r = block_sum(dz, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// y += l d; g += l z (l = alpha/beta); s = M g; partials of g.s and g.g
+__global__ void __launch_bounds__(256)
+    k_pcg_update(PcgParams Q, const uint8_t *oid, const double *P, double *y, double *gv, const double *d,
+                 const double *z, double *partials, int nb, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    double alpha = sc->red[0], beta = sc->red[1];
+    double gs = 0, g2 = 0;
+    if (u < g.nn) {
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
+        int t = node_type(g, oid, u, i, j, k);
+        y[u] = y[u] + (alpha / beta) * d[u];
+        double gg = gv[u] + (alpha / beta) * z[u];
+        gv[u] = gg;
+        double s = (1.0 / j_diag(Q, t, i, j, P, u)) * gg;
+        gs = gg * s;
+        g2 = gg * gg;
+    }
+    double r1 = block_sum(gs, sh);
+    double r2 = block_sum(g2, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = r1;
+        partials[nb + blockIdx.x] = r2;
+    }
+}
+
+// d = (alpha_new/alpha) d - s
+__global__ void __launch_bounds__(256)
+    k_pcg_dir(PcgParams Q, const uint8_t *oid, const double *P, const double *gv, double *d, const DevScalars *sc) {
+    if (sc->done & 2) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    if (u >= g.nn) return;
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
+    int t = node_type(g, oid, u, i, j, k);
+    double s = (1.0 / j_diag(Q, t, i, j, P, u)) * gv[u];
+    double alpha_new = sc->red[3], alpha = sc->red[0];
+    d[u] = (alpha_new / alpha) * d[u] - s;
+}
+
+// scalar steps between the vector kernels (one workgroup each)
+// mode 0: alpha = alpha_new = sum (init)
+// mode 1: beta = sum (after spmv)
+// mode 2: alpha_new = sum0, gg = sum1, convergence test (after update)
+// mode 3: alpha = alpha_new (after dir)
+__global__ void __launch_bounds__(1024)
+    k_pcg_scalar(const double *partials, int nb, int mode, double nu, double tol, int it, DevScalars *sc) {
+    __shared__ double sh[16];
+    if (mode != 0 && (sc->done & 1)) {
+        if (mode == 2 && threadIdx.x == 0) sc->done |= 2;  // converged earlier: freeze d
+        return;
+    }
+    if (mode == 3) {
+        if (threadIdx.x == 0) sc->red[0] = sc->red[3];
+        return;
+    }
+    double s0 = fixed_sum(partials, nb, sh);
+    double s1 = mode == 2 ? fixed_sum(partials + nb, nb, sh) : 0.0;
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        sc->red[0] = s0;
+        sc->red[3] = s0;
+    } else if (mode == 1) {
+        sc->red[1] = s0;
+    } else {
+        sc->red[3] = s0;
+        sc->red[2] = s1;
+        double l2 = sqrt(s1 / nu);  // vec::norm is RMS (ch3/ver2/PotentialSolver.cpp:114-121)
+        sc->red[4] = l2;
+        sc->iters = it + 1;
+        if (l2 < tol) sc->done = 1;
+    }
+}
+
+// red-black Gauss-Seidel (w = 1) on J y = F: the fallback solveGSLinear (:433-461)
+__global__ void __launch_bounds__(256)
+    k_gsl_sweep(PcgParams Q, const uint8_t *oid, const double *P, double *y, const double *F, int color,
+                const DevScalars *sc) {
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    if (u >= g.nn) return;
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
+    if (((i + j + k) & 1) != color) return;
+    int t = node_type(g, oid, u, i, j, k);
+    double jd = j_diag(Q, t, i, j, P, u);
+    double S = row_apply(Q, t, u, i, j, k, jd, y) - jd * y[u];
+    double phi_new = (F[u] - S) / jd;
+    y[u] = y[u] + 1. * (phi_new - y[u]);
+}
+
+__global__ void __launch_bounds__(256)
+    k_gsl_residual(PcgParams Q, const uint8_t *oid, const double *P, const double *y, const double *F,
+                   double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    double r2 = 0;
+    if (u < g.nn) {
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
+        int t = node_type(g, oid, u, i, j, k);
+        double R = row_apply(Q, t, u, i, j, k, j_diag(Q, t, i, j, P, u), y) - F[u];
+        r2 = R * R;
+    }
+    double r = block_sum(r2, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// y = 0 on Dirichlet nodes; x -= y; partials of y.y  (:273-283)
+__global__ void __launch_bounds__(256)
+    k_nr_update(PcgParams Q, const uint8_t *oid, double *x, double *y, double *partials) {
+    __shared__ double sh[4];
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    double y2 = 0;
+    if (u < g.nn) {
+        double yy = y[u];
+        if (oid[u] > 0) {
+            yy = 0;
+            y[u] = 0;
+        }
+        x[u] = x[u] - yy;
+        y2 = yy * yy;
+    }
+    double r = block_sum(y2, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ void k_reset_flags(DevScalars *sc) {
+    sc->done = 0;
+    sc->iters = 0;
+}
+
+__global__ void __launch_bounds__(1024) k_sum_to(const double *partials, int nb, double *out) {
+    __shared__ double sh[16];
+    double s = fixed_sum(partials, nb, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_fill(long n, double *p, double v) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u < n) p[u] = v;
+}
+
+static unsigned nbk(long n) { return (unsigned)((n + 255) / 256); }
+
+static SorParams sor_params(const pic_solver *s) {
+    const Grid &g = s->w->g;
+    SorParams S;
+    S.idx2 = 1.0 / (g.dh[0] * g.dh[0]);
+    S.idy2 = 1.0 / (g.dh[1] * g.dh[1]);
+    S.idz2 = 1.0 / (g.dh[2] * g.dh[2]);
+    S.denom = 2 * S.idx2 + 2 * S.idy2 + 2 * S.idz2;
+    S.w = 1.4;
+    S.phi0 = s->phi0;
+    S.Te0 = s->Te0;
+    S.n0 = s->n0;
+    S.tol = s->tol;
+    S.max_it = s->max_it;
+    return S;
+}
+
+static size_t max_lds_bytes(int dev) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+    return (size_t)v;
+}
+
+static int read_scalars(pic_world *w) {
+    PIC_HIP(hipMemcpyAsync(w->scal_host, w->scal, sizeof(DevScalars), hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipStreamSynchronize(w->stream));
+    return PIC_OK;
+}
+
+// SOR (linear or Boltzmann): returns PIC_OK / PIC_NOT_CONVERGED
+template <bool BOLTZ>
+static int solve_sor(pic_solver *s, double *residual, int *iters) {
+    pic_world *w = s->w;
+    const Grid &g = w->g;
+    SorParams S = sor_params(s);
+    k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
+    size_t lds = (size_t)g.nn * sizeof(double);
+    size_t lim = max_lds_bytes(w->device);
+    bool done_lds = false;
+    if (lds + 1024 <= lim) {
+        static bool attr_set[2] = {false, false};
+        if (!attr_set[BOLTZ]) {
+            hipFuncSetAttribute((const void *)k_sor_lds<BOLTZ>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(lim - 1024));
+            attr_set[BOLTZ] = true;
+        }
+        k_sor_lds<BOLTZ><<<1, LDS_THREADS, lds, w->stream>>>(g, w->phi, w->rho, w->object_id, S, w->scal);
+        if (hipGetLastError() == hipSuccess) done_lds = true;
+    }
+    if (!done_lds) {
+        long half = (g.ni + 1) / 2;
+        unsigned nb_sweep = nbk(half * (long)g.nj * g.nk);
+        unsigned nb_res = nbk(g.nn);
+        int rc = grow_partials(w, (int)nb_res);
+        if (rc) return rc;
+        const int CHUNK = 100;
+        for (int it0 = 0; it0 < S.max_it; it0 += CHUNK) {
+            int it1 = std::min(S.max_it, it0 + CHUNK);
+            for (int it = it0; it < it1; it++) {
+                k_sor_sweep<BOLTZ><<<nb_sweep, 256, 0, w->stream>>>(g, w->phi, w->rho, w->object_id, S, 0, w->scal);
+                k_sor_sweep<BOLTZ><<<nb_sweep, 256, 0, w->stream>>>(g, w->phi, w->rho, w->object_id, S, 1, w->scal);
+                if (it % 25 == 0) {
+                    k_sor_residual<BOLTZ><<<nb_res, 256, 0, w->stream>>>(g, w->phi, w->rho, w->object_id, S,
+                                                                         w->partials, w->scal);
+                    k_sor_check<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb_res,
+                                                          (double)(g.ni * g.nj * g.nk), S.tol, it, w->scal);
+                }
+            }
+            PIC_LAUNCH_CHECK();
+            rc = read_scalars(w);
+            if (rc) return rc;
+            if (w->scal_host->done) break;
+        }
+        if (!w->scal_host->done) w->scal_host->iters = S.max_it;
+    } else {
+        int rc = read_scalars(w);
+        if (rc) return rc;
+    }
+    if (residual) *residual = w->scal_host->red[0];
+    if (iters) *iters = w->scal_host->iters;
+    w->ef_valid = false;
+    return w->scal_host->done ? PIC_OK : PIC_NOT_CONVERGED;
+}
+
+static PcgParams pcg_params(const pic_solver *s) {
+    const Grid &g = s->w->g;
+    PcgParams Q;
+    Q.g = g;
+    Q.idx = 1.0 / g.dh[0];
+    Q.idy = 1.0 / g.dh[1];
+    Q.idz = 1.0 / g.dh[2];
+    Q.idx2 = Q.idx * Q.idx;
+    Q.idy2 = Q.idy * Q.idy;
+    Q.idz2 = Q.idz * Q.idz;
+    Q.phi0 = s->phi0;
+    Q.Te0 = s->Te0;
+    Q.n0 = s->n0;
+    Q.tol = s->tol;
+    return Q;
+}
+
+// solvePCGLinear on J y = F; returns converged and adds iterations to *its
+static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *its) {
+    pic_world *w = s->w;
+    long nn = Q.g.nn;
+    unsigned nb = nbk(nn);
+    int rc = grow_partials(w, (int)nb);
+    if (rc) return rc;
+    double nu = (double)(int)nn;
+    k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
+    k_pcg_init<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, s->g, s->d, w->partials);
+    k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 0, nu, Q.tol, 0, w->scal);
+    const int CHUNK = 32;
+    bool conv = false;
+    for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
+        int it1 = std::min(s->max_it, it0 + CHUNK);
+        for (int it = it0; it < it1; it++) {
+            k_pcg_spmv<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->d, s->z, w->partials, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 1, nu, Q.tol, it, w->scal);
+            k_pcg_update<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->g, s->d, s->z, w->partials,
+                                                   (int)nb, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 2, nu, Q.tol, it, w->scal);
+            k_pcg_dir<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, s->d, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 3, nu, Q.tol, it, w->scal);
+        }
+        PIC_LAUNCH_CHECK();
+        rc = read_scalars(w);
+        if (rc) return rc;
+        if (w->scal_host->done & 1) {
+            conv = true;
+            break;
+        }
+    }
+    *its += conv ? w->scal_host->iters : s->max_it;
+    *converged = conv;
+    return PIC_OK;
+}
+
+static int gs_linear(pic_solver *s, const PcgParams &Q) {
+    pic_world *w = s->w;
+    long nn = Q.g.nn;
+    unsigned nb = nbk(nn);
+    k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
+    const int CHUNK = 100;
+    for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
+        int it1 = std::min(s->max_it, it0 + CHUNK);
+        for (int it = it0; it < it1; it++) {
+            k_gsl_sweep<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, 0, w->scal);
+            k_gsl_sweep<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, 1, w->scal);
+            if (it % 25 == 0) {
+                k_gsl_residual<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, w->partials, w->scal);
+                k_sor_check<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, (double)(int)nn, Q.tol, it, w->scal);
+            }
+        }
+        PIC_LAUNCH_CHECK();
+        int rc = read_scalars(w);
+        if (rc) return rc;
+        if (w->scal_host->done) break;
+    }
+    return PIC_OK;
+}
+
+static int ensure_pcg_vectors(pic_solver *s) {
+    if (s->y) return PIC_OK;
+    long nn = s->w->g.nn;
+    double **v[6] = {&s->y, &s->g, &s->d, &s->z, &s->F, &s->P};
+    for (auto p : v) PIC_HIP(hipMalloc(p, sizeof(double) * nn));
+    return PIC_OK;
+}
+
+static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
+    pic_world *w = s->w;
+    int rc = ensure_pcg_vectors(s);
+    if (rc) return rc;
+    PcgParams Q = pcg_params(s);
+    long nn = w->g.nn;
+    unsigned nb = nbk(nn);
+    rc = grow_partials(w, (int)nb);
+    if (rc) return rc;
+    // y starts at zero for each solve and is warm-started across Newton steps (:232)
+    k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
+    k_fill<<<nb, 256, 0, w->stream>>>(nn, s->P, 0.0);
+    const int NR_MAX_IT = 20;
+    const double NR_TOL = 1e-3;
+    bool converged = false;
+    double norm = 0;
+    int pcg_its = 0;
+    for (int it = 0; it < NR_MAX_IT; it++) {
+        k_nr_F<<<nb, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P);
+        PIC_LAUNCH_CHECK();
+        bool pc = false;
+        rc = pcg_linear(s, Q, &pc, &pcg_its);
+        if (rc) return rc;
+        if (!pc) {
+            rc = gs_linear(s, Q);
+            if (rc) return rc;
+        }
+        k_nr_update<<<nb, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
+        k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, &w->scal->red[5]);
+        PIC_LAUNCH_CHECK();
+        rc = read_scalars(w);
+        if (rc) return rc;
+        norm = sqrt(w->scal_host->red[5] / (double)(int)nn);
+        if (norm < NR_TOL) {
+            converged = true;
+            break;
+        }
+    }
+    if (residual) *residual = norm;
+    if (iters) *iters = pcg_its;
+    w->ef_valid = false;
+    return converged ? PIC_OK : PIC_NOT_CONVERGED;
+}
+
+}  // namespace pic
+
+using namespace pic;
+
+extern "C" {
+
+int pic_solver_create(pic_world *w, int type, int max_it, double tol, pic_solver **out) {
+    PIC_CHECK(w && out, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(type >= PIC_SOLVER_GS && type <= PIC_SOLVER_SOR_LINEAR, PIC_ERR_ARG, "bad solver type");
+    PIC_CHECK(max_it >= 0, PIC_ERR_ARG, "negative max_it");
+    PIC_CHECK(w->extents_set, PIC_ERR_STATE, "setExtents must precede the solver");
+    pic_solver *s = new pic_solver();
+    s->w = w;
+    s->type = type;
+    s->max_it = max_it;
+    s->tol = tol;
+    *out = s;
+    return PIC_OK;
+}
+
+int pic_solver_destroy(pic_solver *s) {
+    if (!s) return PIC_OK;
+    if (s->w && s->w->stream) hipStreamSynchronize(s->w->stream);
+    hipFree(s->y);
+    hipFree(s->g);
+    hipFree(s->d);
+    hipFree(s->z);
+    hipFree(s->F);
+    hipFree(s->P);
+    delete s;
+    return PIC_OK;
+}
+
+int pic_solver_set_reference_values(pic_solver *s, double phi0, double Te0, double n0) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(Te0 > 0 && n0 > 0, PIC_ERR_ARG, "Te0 and n0 must be positive");
+    s->phi0 = phi0;
+    s->Te0 = Te0;
+    s->n0 = n0;
+    return PIC_OK;
+}
+
+int pic_solver_solve(pic_solver *s, double *residual, int *iters) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    pic_world *w = s->w;
+    hipEvent_t ev;
+    timer_begin(w, PIC_T_SOLVE, &ev);
+    int rc;
+    switch (s->type) {
+    case PIC_SOLVER_SOR_LINEAR: rc = solve_sor<false>(s, residual, iters); break;
+    case PIC_SOLVER_GS: rc = solve_sor<true>(s, residual, iters); break;
+    case PIC_SOLVER_PCG: rc = solve_nrpcg(s, residual, iters); break;
+    default:
+        rc = solve_qn(w, s->phi0, s->Te0, s->n0);
+        if (residual) *residual = 0;
+        if (iters) *iters = 1;
+        break;
+    }
+    timer_end(w, PIC_T_SOLVE, ev);
+    return rc;
+}
+
+int pic_solver_solve_qn(pic_solver *s) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    return solve_qn(s->w, s->phi0, s->Te0, s->n0);
+}
+
+int pic_solver_compute_ef(pic_solver *s) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    int rc = compute_ef(s->w, true, true);
+    if (rc) return rc;
+    s->w->ef_valid = true;
+    return PIC_OK;
+}
+
+}  // extern "C"

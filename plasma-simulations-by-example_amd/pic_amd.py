@@ -1,0 +1,357 @@
+"""Python binding of libpicamd (ctypes over the C ABI in include/picamd.h).
+
+Mirrors the reference class API (ch3/ver2/World.h, Species.h, PotentialSolver.h):
+``World``, ``Species``, ``PotentialSolver``.  Every call goes to the HIP library; there is
+no CPU fallback -- without the built library or a GPU the constructors raise.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpicamd.so")
+
+PIC_OK, PIC_NOT_CONVERGED = 0, 1
+MIXED, FP64 = 0, 1
+REFLECT, ABSORB = 0, 1
+GS, PCG, QN, SOR_LINEAR = 0, 1, 2, 3
+PHI, RHO, NODE_VOL, EF, OBJECT_ID = 0, 1, 2, 3, 4
+T_PUSH, T_DEPOSIT, T_SORT, T_SOLVE, T_EF, T_RHO, T_INJECT = range(7)
+T_NAMES = ["push", "deposit", "sort", "solve", "ef", "rho", "inject"]
+
+# every symbol include/picamd.h declares
+EXPORTS = [
+    "pic_abi_version", "pic_last_error", "pic_device_count",
+    "pic_world_create", "pic_world_destroy", "pic_world_set_extents", "pic_world_add_sphere",
+    "pic_world_add_inlet", "pic_world_get_field", "pic_world_set_field",
+    "pic_world_compute_charge_density", "pic_world_get_pe", "pic_world_sync", "pic_world_stream",
+    "pic_world_set_timing", "pic_world_get_timers",
+    "pic_species_create", "pic_species_destroy", "pic_species_add_particles",
+    "pic_species_set_particles", "pic_species_get_particles", "pic_species_get_np",
+    "pic_species_advance", "pic_species_compute_number_density", "pic_species_get_den",
+    "pic_species_get_den_ave", "pic_species_update_averages", "pic_species_diag",
+    "pic_species_sort", "pic_species_inject_beam", "pic_species_load_box_device",
+    "pic_solver_create", "pic_solver_destroy", "pic_solver_set_reference_values",
+    "pic_solver_solve", "pic_solver_solve_qn", "pic_solver_compute_ef",
+]
+
+_dp = C.POINTER(C.c_double)
+_lp = C.POINTER(C.c_long)
+_u32p = C.POINTER(C.c_uint32)
+_vp = C.c_void_p
+
+
+class PicError(RuntimeError):
+    pass
+
+
+_lib = None
+_exiting = False
+
+
+def _at_exit():
+    # objects still alive at interpreter exit are not released through the library: the
+    # HIP runtime may already be tearing down, and the process frees the device anyway
+    global _exiting
+    _exiting = True
+
+
+atexit.register(_at_exit)
+
+
+def build():
+    """Compiles libpicamd.so (hipcc, gfx950) in the package directory."""
+    subprocess.run(["make", "-s", "-C", HERE, "lib"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PicError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "pic_abi_version": (C.c_int, []),
+            "pic_last_error": (C.c_char_p, []),
+            "pic_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+            "pic_world_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+            "pic_world_destroy": (C.c_int, [_vp]),
+            "pic_world_set_extents": (C.c_int, [_vp, _dp, _dp]),
+            "pic_world_add_sphere": (C.c_int, [_vp, _dp, C.c_double, C.c_double]),
+            "pic_world_add_inlet": (C.c_int, [_vp]),
+            "pic_world_get_field": (C.c_int, [_vp, C.c_int, _dp]),
+            "pic_world_set_field": (C.c_int, [_vp, C.c_int, _dp]),
+            "pic_world_compute_charge_density": (C.c_int, [_vp, C.POINTER(_vp), C.c_int]),
+            "pic_world_get_pe": (C.c_int, [_vp, _dp]),
+            "pic_world_sync": (C.c_int, [_vp]),
+            "pic_world_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+            "pic_world_set_timing": (C.c_int, [_vp, C.c_int]),
+            "pic_world_get_timers": (C.c_int, [_vp, _dp, _lp, C.c_int]),
+            "pic_species_create": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                             C.c_long, C.c_int, C.POINTER(_vp)]),
+            "pic_species_destroy": (C.c_int, [_vp]),
+            "pic_species_add_particles": (C.c_int, [_vp, C.c_long, _dp, _dp, _dp, C.c_double,
+                                                    _lp]),
+            "pic_species_set_particles": (C.c_int, [_vp, C.c_long, _dp, _dp, _dp, _u32p]),
+            "pic_species_get_particles": (C.c_int, [_vp, _dp, _dp, _dp, _u32p]),
+            "pic_species_get_np": (C.c_int, [_vp, _lp]),
+            "pic_species_advance": (C.c_int, [_vp, C.c_int, C.c_double]),
+            "pic_species_compute_number_density": (C.c_int, [_vp]),
+            "pic_species_get_den": (C.c_int, [_vp, _dp]),
+            "pic_species_get_den_ave": (C.c_int, [_vp, _dp]),
+            "pic_species_update_averages": (C.c_int, [_vp]),
+            "pic_species_diag": (C.c_int, [_vp, _dp]),
+            "pic_species_sort": (C.c_int, [_vp]),
+            "pic_species_inject_beam": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double,
+                                                  C.c_double, C.c_uint64, C.c_uint64, _lp]),
+            "pic_species_load_box_device": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_long,
+                                                      C.c_double, C.c_double, C.c_uint64]),
+            "pic_solver_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.POINTER(_vp)]),
+            "pic_solver_destroy": (C.c_int, [_vp]),
+            "pic_solver_set_reference_values": (C.c_int, [_vp, C.c_double, C.c_double,
+                                                          C.c_double]),
+            "pic_solver_solve": (C.c_int, [_vp, _dp, C.POINTER(C.c_int)]),
+            "pic_solver_solve_qn": (C.c_int, [_vp]),
+            "pic_solver_compute_ef": (C.c_int, [_vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, allow_nc=False):
+    if rc == PIC_OK or (allow_nc and rc == PIC_NOT_CONVERGED):
+        return rc
+    msg = lib().pic_last_error().decode(errors="replace")
+    raise PicError(f"libpicamd error {rc}: {msg}")
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _v3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().pic_device_count(C.byref(n)))
+    return n.value
+
+
+class World:
+    """World(ni, nj, nk) with node fields on the GPU (ch3/ver2/World.h:38-147)."""
+
+    def __init__(self, ni, nj, nk):
+        self.h = _vp()
+        _check(lib().pic_world_create(ni, nj, nk, C.byref(self.h)))
+        self.ni, self.nj, self.nk = ni, nj, nk
+        self.nn = ni * nj * nk
+        self.x0 = self.xm = self.dh = None
+
+    def close(self):
+        if self.h and not _exiting:
+            lib().pic_world_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_extents(self, x0, xm):
+        x0 = _v3(x0)
+        xm = _v3(xm)
+        _check(lib().pic_world_set_extents(self.h, _d(x0), _d(xm)))
+        self.x0, self.xm = x0, xm
+        self.dh = (xm - x0) / (np.array([self.ni, self.nj, self.nk]) - 1)
+
+    def add_sphere(self, c, r, phi_s):
+        _check(lib().pic_world_add_sphere(self.h, _d(_v3(c)), r, phi_s))
+
+    def add_inlet(self):
+        _check(lib().pic_world_add_inlet(self.h))
+
+    def field(self, which):
+        out = np.zeros(self.nn * (3 if which == EF else 1))
+        _check(lib().pic_world_get_field(self.h, which, _d(out)))
+        return out
+
+    def set_field(self, which, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.nn * (3 if which == EF else 1)
+        _check(lib().pic_world_set_field(self.h, which, _d(v)))
+
+    def compute_charge_density(self, species):
+        arr = (_vp * len(species))(*[s.h for s in species])
+        _check(lib().pic_world_compute_charge_density(self.h, arr, len(species)))
+
+    def get_pe(self):
+        pe = C.c_double(0)
+        _check(lib().pic_world_get_pe(self.h, C.byref(pe)))
+        return pe.value
+
+    def sync(self):
+        _check(lib().pic_world_sync(self.h))
+
+    def stream(self):
+        s = _vp()
+        _check(lib().pic_world_stream(self.h, C.byref(s)))
+        return s.value
+
+    def set_timing(self, on=True):
+        _check(lib().pic_world_set_timing(self.h, 1 if on else 0))
+
+    def timers(self, reset=False):
+        ms = np.zeros(7)
+        calls = np.zeros(7, dtype=np.int64)
+        _check(lib().pic_world_get_timers(self.h, _d(ms), calls.ctypes.data_as(_lp),
+                                          1 if reset else 0))
+        return {T_NAMES[i]: (ms[i], int(calls[i])) for i in range(7)}
+
+
+class Species:
+    """Species(name, mass, charge, mpw0, world) (ch3/ver2/Species.h:22-27)."""
+
+    def __init__(self, world: World, mass, charge, mpw0=0.0, precision=MIXED, capacity=1 << 16,
+                 track_ids=False, name=""):
+        self.world = world
+        self.name = name
+        self.mass, self.charge, self.mpw0 = mass, charge, mpw0
+        self.precision = precision
+        self.track_ids = track_ids
+        self.h = _vp()
+        _check(lib().pic_species_create(world.h, mass, charge, mpw0, precision, int(capacity),
+                                        1 if track_ids else 0, C.byref(self.h)))
+
+    def close(self):
+        if self.h and not _exiting:
+            lib().pic_species_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_particles(self, pos, vel, mpw, dt):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        mpw = np.ascontiguousarray(mpw, dtype=np.float64).reshape(-1)
+        n = C.c_long(0)
+        _check(lib().pic_species_add_particles(self.h, len(mpw), _d(pos), _d(vel), _d(mpw), dt,
+                                               C.byref(n)))
+        return n.value
+
+    def set_particles(self, pos, vel, mpw, ids=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        mpw = np.ascontiguousarray(mpw, dtype=np.float64).reshape(-1)
+        idp = None
+        if ids is not None:
+            ids = np.ascontiguousarray(ids, dtype=np.uint32)
+            idp = ids.ctypes.data_as(_u32p)
+        _check(lib().pic_species_set_particles(self.h, len(mpw), _d(pos), _d(vel), _d(mpw), idp))
+
+    def np(self):
+        n = C.c_long(0)
+        _check(lib().pic_species_get_np(self.h, C.byref(n)))
+        return n.value
+
+    def particles(self):
+        """(pos (n,3), vel (n,3), mpw (n,), ids or None) in device order."""
+        n = self.np()
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        mpw = np.zeros(n)
+        ids = np.zeros(n, dtype=np.uint32) if self.track_ids else None
+        if n:
+            _check(lib().pic_species_get_particles(
+                self.h, _d(pos), _d(vel), _d(mpw),
+                ids.ctypes.data_as(_u32p) if ids is not None else None))
+        return pos, vel, mpw, ids
+
+    def advance(self, bc, dt):
+        _check(lib().pic_species_advance(self.h, bc, dt))
+
+    def compute_number_density(self):
+        _check(lib().pic_species_compute_number_density(self.h))
+
+    def den(self):
+        out = np.zeros(self.world.nn)
+        _check(lib().pic_species_get_den(self.h, _d(out)))
+        return out
+
+    def den_ave(self):
+        out = np.zeros(self.world.nn)
+        _check(lib().pic_species_get_den_ave(self.h, _d(out)))
+        return out
+
+    def update_averages(self):
+        _check(lib().pic_species_update_averages(self.h))
+
+    def diag(self):
+        """[real_count, px, py, pz, KE]"""
+        out = np.zeros(5)
+        _check(lib().pic_species_diag(self.h, _d(out)))
+        return out
+
+    def sort(self):
+        _check(lib().pic_species_sort(self.h))
+
+    def inject_beam(self, v_drift, den, T, dt, seed, counter):
+        n = C.c_long(0)
+        _check(lib().pic_species_inject_beam(self.h, v_drift, den, T, dt, seed, counter,
+                                             C.byref(n)))
+        return n.value
+
+    def load_box_device(self, x1, x2, num_den, num_mp, v_drift=0.0, T=0.0, seed=1):
+        _check(lib().pic_species_load_box_device(self.h, _d(_v3(x1)), _d(_v3(x2)), num_den,
+                                                 int(num_mp), v_drift, T, seed))
+
+
+class PotentialSolver:
+    """PotentialSolver(world, type, max_it, tol) (ch3/ver2/PotentialSolver.h:46-50)."""
+
+    def __init__(self, world: World, stype=GS, max_it=10000, tol=1e-4):
+        self.world = world
+        self.h = _vp()
+        _check(lib().pic_solver_create(world.h, stype, max_it, tol, C.byref(self.h)))
+
+    def close(self):
+        if self.h and not _exiting:
+            lib().pic_solver_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_reference_values(self, phi0, Te0, n0):
+        _check(lib().pic_solver_set_reference_values(self.h, phi0, Te0, n0))
+
+    def solve(self):
+        """Returns (converged, residual, iterations)."""
+        r = C.c_double(0)
+        it = C.c_int(0)
+        rc = _check(lib().pic_solver_solve(self.h, C.byref(r), C.byref(it)), allow_nc=True)
+        return rc == PIC_OK, r.value, it.value
+
+    def solve_qn(self):
+        _check(lib().pic_solver_solve_qn(self.h))
+
+    def compute_ef(self):
+        _check(lib().pic_solver_compute_ef(self.h))
