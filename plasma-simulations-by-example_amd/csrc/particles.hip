@@ -1303,7 +1303,7 @@ static void free_buf(PartBuf &b) {
     b = PartBuf{};
 }
 
-static long default_stage_cap(long cap_bins) { return std::max<long>(65536, cap_bins / 8); }
+static long default_stage_cap(long cap_bins) { return std::max<long>(65536, cap_bins / 16); }
 
 static int launch_plan_push(pic_species *s, uint32_t *err) {
     pic_world *w = s->w;
@@ -1435,7 +1435,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     s->track_ids = track_ids != 0;
     const Grid &g = w->g;
     long cap = std::max<long>(capacity, 1024);
-    long cap_bins = cap + cap / 2 + (long)g.ntiles * 48;
+    long cap_bins = cap + (cap * 3) / 10 + (long)g.ntiles * 64;
     long cap_stage = default_stage_cap(cap_bins);
     int rc = PIC_OK;
     size_t wbytes = (size_t)g.ntiles * WNODES * (precision == PIC_MIXED ? 4 : 8);
