@@ -20,6 +20,7 @@ namespace pic {
 int compute_ef(pic_world *w, bool want32, bool want64);
 
 constexpr int NSLOT_DIAG = 5;
+constexpr int NDIAG_SLOTS = NSLOT_DIAG + 1;  // + particle count
 
 // ---------------------------------------------------------------------------------------
 // push parameters
@@ -547,8 +548,7 @@ struct PushOut {
     double *win64;
     uint8_t *win_used;
     double *den_extra;
-    long *np_dev;
-    double *diag;
+    double *diag_part;  // [NDIAG_SLOTS][gridDim.x] per-block partials, reduced on demand
     uint32_t *err;
 };
 
@@ -593,6 +593,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         uint32_t cnt = clamped_count(in.bin_count, in.bin_cap, t);
         if (cnt == 0) {
             if (tid == 0) O.win_used[t] = 0;
+            if (tid < NDIAG_SLOTS) O.diag_part[tid * gridDim.x + blockIdx.x] = 0.0;
             return;
         }
         uint32_t start = in.bin_start[t];
@@ -793,14 +794,11 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         if ((tid & 63) == 0) sRed[NSLOT_DIAG][tid >> 6] = v;
     }
     __syncthreads();
-    if (tid < NSLOT_DIAG + 1) {
+    // one partial per block and slot: no two blocks write the same address
+    if (tid < NDIAG_SLOTS) {
         double s = 0;
         for (int i = 0; i < PUSH_THREADS / 64; i++) s += sRed[tid][i];
-        if (tid < NSLOT_DIAG) {
-            if (s != 0) atomicAdd(&O.diag[tid], s);
-        } else if (s != 0) {
-            atomicAdd((unsigned long long *)O.np_dev, (unsigned long long)s);
-        }
+        O.diag_part[tid * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -879,7 +877,7 @@ __global__ void __launch_bounds__(256) k_deposit_all(Grid g, PartView in, double
 
 // diagnostics over all particles (when the fused ones are stale)
 template <int PREC>
-__global__ void __launch_bounds__(256) k_diag_all(Grid g, PartView in, double *diag, long *np) {
+__global__ void __launch_bounds__(256) k_diag_all(Grid g, PartView in, double *diag_part) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
     __shared__ double sh[8];
     double acc[NSLOT_DIAG] = {0, 0, 0, 0, 0};
@@ -913,10 +911,21 @@ __global__ void __launch_bounds__(256) k_diag_all(Grid g, PartView in, double *d
     }
     for (int q = 0; q < NSLOT_DIAG; q++) {
         double s = block_sum(acc[q], sh);
-        if (threadIdx.x == 0 && s != 0) atomicAdd(&diag[q], s);
+        if (threadIdx.x == 0) diag_part[q * gridDim.x + blockIdx.x] = s;
     }
     double s = block_sum(cnt_all, sh);
-    if (threadIdx.x == 0 && s != 0) atomicAdd((unsigned long long *)np, (unsigned long long)s);
+    if (threadIdx.x == 0) diag_part[NSLOT_DIAG * gridDim.x + blockIdx.x] = s;
+}
+
+// sums nslot rows of nb per-block partials in a fixed order (one workgroup)
+__global__ void __launch_bounds__(1024) k_reduce_slots(const double *part, int nb, int nslot, double *out) {
+    __shared__ double sh[16];
+    for (int q = 0; q < nslot; q++) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long)q * nb + b];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) out[q] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1310,7 +1319,7 @@ static int launch_plan_push(pic_species *s, uint32_t *err) {
     PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
     k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(w->g, in.bin_count, in.bin_cap, nullptr, 0, out.bin_start,
                                               out.bin_cap, out.bin_count, out.stage_count, out.cap_bins,
-                                              s->np_dev, s->diag_dev, err);
+                                              nullptr, nullptr, err);
     PIC_LAUNCH_CHECK();
     return PIC_OK;
 }
@@ -1457,6 +1466,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMalloc(&s->win, wbytes));
     TRY(hipMalloc(&s->win_used, g.ntiles));
     TRY(hipMalloc(&s->diag_dev, sizeof(double) * 8));
+    TRY(hipMalloc(&s->diag_part, sizeof(double) * NDIAG_SLOTS * (size_t)(g.ntiles + STAGE_WGS)));
     TRY(hipHostMalloc(&s->diag_host, sizeof(double) * 8));
     TRY(hipMemsetAsync(s->np_dev, 0, sizeof(long), w->stream));
     TRY(hipMemsetAsync(s->den, 0, sizeof(double) * g.nn, w->stream));
@@ -1489,6 +1499,7 @@ int pic_species_destroy(pic_species *s) {
     hipFree(s->win);
     hipFree(s->win_used);
     hipFree(s->diag_dev);
+    hipFree(s->diag_part);
     if (s->diag_host) hipHostFree(s->diag_host);
     delete s;
     return PIC_OK;
@@ -1705,8 +1716,7 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     O.win64 = (double *)s->win;
     O.win_used = s->win_used;
     O.den_extra = s->den_extra;
-    O.np_dev = s->np_dev;
-    O.diag = s->diag_dev;
+    O.diag_part = s->diag_part;
     O.err = err_flag(s);
     unsigned grid = g.ntiles + STAGE_WGS;
     hipEvent_t ev;
@@ -1779,18 +1789,18 @@ int pic_species_update_averages(pic_species *s) {
 int pic_species_diag(pic_species *s, double out[5]) {
     PIC_CHECK(s && out, PIC_ERR_ARG, "null argument");
     pic_world *w = s->w;
+    unsigned grid = w->g.ntiles + STAGE_WGS;
     if (!s->diag_valid) {
-        k_fill_f64<<<1, 64, 0, w->stream>>>(8, s->diag_dev, 0.0);
-        PIC_HIP(hipMemsetAsync(s->np_dev, 0, sizeof(long), w->stream));
-        unsigned grid = w->g.ntiles + STAGE_WGS;
         PartView v = view_of(s->buf[s->cur]);
         if (s->prec == PIC_MIXED)
-            k_diag_all<PIC_MIXED><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_dev, s->np_dev);
+            k_diag_all<PIC_MIXED><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_part);
         else
-            k_diag_all<PIC_FP64><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_dev, s->np_dev);
+            k_diag_all<PIC_FP64><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_part);
         PIC_LAUNCH_CHECK();
         s->diag_valid = true;
     }
+    k_reduce_slots<<<1, 1024, 0, w->stream>>>(s->diag_part, (int)grid, NDIAG_SLOTS, s->diag_dev);
+    PIC_LAUNCH_CHECK();
     PIC_HIP(hipMemcpyAsync(s->diag_host, s->diag_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, w->stream));
     PIC_HIP(hipStreamSynchronize(w->stream));
     // getRealCount, getMomentum = mass*sum(v*mpw), getKE = 0.5*mass*sum(mpw*v^2)

@@ -126,6 +126,7 @@ struct pic_world {
     pic::DevScalars *scal_host = nullptr;  // pinned mirror
     double *partials = nullptr;       // per-block reduction partials
     int partials_cap = 0;
+    double *pe_part = nullptr;        // per-block |E|^2 V partials of the last computeEF
     bool ef_valid = false;            // ef32/ef64 hold computeEF of the current phi
     bool pe_valid = false;
     // timers
@@ -151,7 +152,8 @@ struct pic_species {
     int ave_samples = 0;
     void *win = nullptr;          // per-tile deposit windows (float or double)
     uint8_t *win_used = nullptr;
-    double *diag_dev = nullptr;   // [5] fused diagnostics of the last push
+    double *diag_dev = nullptr;   // [6] reduced diagnostics
+    double *diag_part = nullptr;  // per-block partials of the last push (or diag pass)
     double *diag_host = nullptr;  // pinned
     bool diag_valid = false;      // diag_dev matches the current particles
     int den_state = 0;            // 0 stale, 1 windows hold the last push deposit, 2 den final

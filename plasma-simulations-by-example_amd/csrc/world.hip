@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_compute_ef(Grid g, const double *__rest
         pe = (ex * ex + ey * ey + ez * ez) * V;
     }
     double s = block_sum(pe, sh);
-    if (threadIdx.x == 0 && s != 0) atomicAdd(pe_acc, s);
+    if (threadIdx.x == 0) pe_acc[blockIdx.x] = s;  // per-block partial, summed by get_pe
 }
 
 // World::getPE over a stored E (after a host write of ef)
@@ -179,7 +179,15 @@ __global__ void __launch_bounds__(256) k_pe_from_ef(Grid g, const double4 *ef64,
         pe = (e.x * e.x + e.y * e.y + e.z * e.z) * V;
     }
     double s = block_sum(pe, sh);
-    if (threadIdx.x == 0 && s != 0) atomicAdd(pe_acc, s);
+    if (threadIdx.x == 0) pe_acc[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(1024) k_sum_partials(const double *part, long nb, double *out) {
+    __shared__ double sh[16];
+    double s = 0;
+    for (long b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = s;
 }
 
 // solveQN (ch3/ver2/PotentialSolver.cpp:204-222)
@@ -239,14 +247,13 @@ int compute_ef(pic_world *w, bool want32, bool want64) {
               "computeEF needs at least 3 nodes per axis");
     hipEvent_t ev;
     timer_begin(w, PIC_T_EF, &ev);
-    PIC_HIP(hipMemsetAsync(&w->scal->pe, 0, sizeof(double), w->stream));
     unsigned nb = nblocks(g.nn, 256);
     if (want32 && want64)
-        k_compute_ef<true, true><<<nb, 256, 0, w->stream>>>(g, w->phi, w->ef32, w->ef64, &w->scal->pe);
+        k_compute_ef<true, true><<<nb, 256, 0, w->stream>>>(g, w->phi, w->ef32, w->ef64, w->pe_part);
     else if (want32)
-        k_compute_ef<true, false><<<nb, 256, 0, w->stream>>>(g, w->phi, w->ef32, w->ef64, &w->scal->pe);
+        k_compute_ef<true, false><<<nb, 256, 0, w->stream>>>(g, w->phi, w->ef32, w->ef64, w->pe_part);
     else
-        k_compute_ef<false, true><<<nb, 256, 0, w->stream>>>(g, w->phi, w->ef32, w->ef64, &w->scal->pe);
+        k_compute_ef<false, true><<<nb, 256, 0, w->stream>>>(g, w->phi, w->ef32, w->ef64, w->pe_part);
     PIC_LAUNCH_CHECK();
     timer_end(w, PIC_T_EF, ev);
     w->pe_valid = true;
@@ -324,6 +331,8 @@ int pic_world_create(int ni, int nj, int nk, pic_world **out) {
     TRY(hipMalloc(&w->ef32, sizeof(float4) * g.nn));
     TRY(hipMalloc(&w->ef64, sizeof(double4) * g.nn));
     TRY(hipMalloc(&w->object_id, g.nn));
+    TRY(hipMalloc(&w->pe_part, sizeof(double) * ((g.nn + 255) / 256)));
+    TRY(hipMemsetAsync(w->pe_part, 0, sizeof(double) * ((g.nn + 255) / 256), w->stream));
     TRY(hipMalloc(&w->scal, sizeof(DevScalars)));
     TRY(hipHostMalloc(&w->scal_host, sizeof(DevScalars)));
     TRY(hipMemsetAsync(w->phi, 0, sizeof(double) * g.nn, w->stream));
@@ -359,6 +368,7 @@ int pic_world_destroy(pic_world *w) {
     hipFree(w->object_id);
     hipFree(w->scal);
     hipFree(w->partials);
+    hipFree(w->pe_part);
     if (w->scal_host) hipHostFree(w->scal_host);
     if (w->stream) hipStreamDestroy(w->stream);
     delete w;
@@ -489,11 +499,12 @@ int pic_world_get_pe(pic_world *w, double *pe) {
     PIC_CHECK(w && pe, PIC_ERR_ARG, "null argument");
     if (!w->pe_valid) {
         // E was written from the host: sum |E|^2 V over the stored field
-        PIC_HIP(hipMemsetAsync(&w->scal->pe, 0, sizeof(double), w->stream));
-        k_pe_from_ef<<<nblocks(w->g.nn, 256), 256, 0, w->stream>>>(w->g, w->ef64, &w->scal->pe);
+        k_pe_from_ef<<<nblocks(w->g.nn, 256), 256, 0, w->stream>>>(w->g, w->ef64, w->pe_part);
         PIC_LAUNCH_CHECK();
         w->pe_valid = true;
     }
+    k_sum_partials<<<1, 1024, 0, w->stream>>>(w->pe_part, (long)nblocks(w->g.nn, 256), &w->scal->pe);
+    PIC_LAUNCH_CHECK();
     PIC_HIP(hipMemcpyAsync(&w->scal_host->pe, &w->scal->pe, sizeof(double), hipMemcpyDeviceToHost,
                            w->stream));
     PIC_HIP(hipStreamSynchronize(w->stream));
