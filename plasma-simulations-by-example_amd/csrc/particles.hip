@@ -433,46 +433,49 @@ __device__ inline void deposit_lds(T *sW, int wi, int wj, int wk, T ox, T oy, T 
 // ---------------------------------------------------------------------------------------
 // per-step bin plan: capacities of the output bins from the input counts
 // ---------------------------------------------------------------------------------------
-// One workgroup of 1024 threads.  cap[t] = need[t] + slack[t], slack = max(n/8, nb/4) + 32
-// where nb is the largest face-neighbour population (room for one step of inflow); the
-// slack is scaled down uniformly when the buffer is short.  Writes starts (exclusive scan),
-// zeroes the output counts and the step's counters.
+// One workgroup of 1024 threads.  Tile t gets an own segment of exactly n[t] slots (its
+// input count: every particle that stays fits) followed by an inflow segment of
+// max(n/8, nb/4) + 32 slots, nb being the largest face-neighbour population (one step of
+// arrivals).  The inflow room is scaled down uniformly when the buffer is short.  Mode 1
+// (counting sort) sizes bins from a histogram with no inflow segment.
 constexpr int PLAN_THREADS = 1024;
 
-__device__ inline uint32_t clamped_count(const uint32_t *cnt, const uint32_t *cap, int t) {
-    uint32_t c = cnt[t], k = cap[t];
-    return c < k ? c : k;
+__device__ inline uint32_t tile_count(const PartView &v, int t) {
+    BinSeg b = bin_of(v, t);
+    return b.c0 + b.c1;
 }
 
 __global__ void __launch_bounds__(PLAN_THREADS)
-    k_plan(Grid g, const uint32_t *cnt_in, const uint32_t *cap_in, const uint32_t *hist, int mode,
-           uint32_t *start_out, uint32_t *cap_out, uint32_t *count_out, uint32_t *stage_count_out,
-           long cap_bins, long *np_dev, double *diag, uint32_t *err) {
+    k_plan(Grid g, PartView in, const uint32_t *hist, int mode, PartView out, uint32_t *err) {
     __shared__ unsigned long long s_need[PLAN_THREADS / 64], s_slack[PLAN_THREADS / 64];
     __shared__ unsigned long long s_scan[PLAN_THREADS];
     int nt = g.ntiles;
     int per = (nt + PLAN_THREADS - 1) / PLAN_THREADS;
     int t0 = threadIdx.x * per, t1 = min(nt, t0 + per);
-    unsigned long long need = 0, slack = 0;
-    for (int t = t0; t < t1; t++) {
-        uint32_t n, nb = 0;
+    auto sizes = [&](int t, uint32_t &n, uint32_t &s) {
         if (mode == 0) {
-            n = clamped_count(cnt_in, cap_in, t);
+            n = tile_count(in, t);
+            uint32_t nb = 0;
             int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
-            if (tx > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - 1));
-            if (tx < g.ntx - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + 1));
-            if (ty > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx));
-            if (ty < g.nty - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx));
-            if (tz > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx * g.nty));
-            if (tz < g.ntz - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx * g.nty));
+            if (tx > 0) nb = max(nb, tile_count(in, t - 1));
+            if (tx < g.ntx - 1) nb = max(nb, tile_count(in, t + 1));
+            if (ty > 0) nb = max(nb, tile_count(in, t - g.ntx));
+            if (ty < g.nty - 1) nb = max(nb, tile_count(in, t + g.ntx));
+            if (tz > 0) nb = max(nb, tile_count(in, t - g.ntx * g.nty));
+            if (tz < g.ntz - 1) nb = max(nb, tile_count(in, t + g.ntx * g.nty));
+            s = max(n / 8, nb / 4) + 32;
         } else {
             n = hist[t];
+            s = 0;
         }
+    };
+    unsigned long long need = 0, slack = 0;
+    for (int t = t0; t < t1; t++) {
+        uint32_t n, s;
+        sizes(t, n, s);
         need += n;
-        uint32_t s = mode == 0 ? max(n / 8, nb / 4) + 32 : n / 8 + 32;
         slack += s;
     }
-    // block sums
     unsigned long long a = need, b = slack;
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_down(a, off, 64);
@@ -488,36 +491,25 @@ __global__ void __launch_bounds__(PLAN_THREADS)
         tot_need += s_need[i];
         tot_slack += s_slack[i];
     }
+    const unsigned long long capb = (unsigned long long)out.cap_bins;
+    bool short_buf = tot_need > capb;
     double alpha = 1.0;
-    if (tot_need > (unsigned long long)cap_bins) {
+    if (short_buf) {
         alpha = 0.0;
-        if (threadIdx.x == 0) atomicOr(err, 1u);  // caller grows the buffers
-    } else if (tot_need + tot_slack > (unsigned long long)cap_bins) {
-        alpha = (double)((unsigned long long)cap_bins - tot_need) / (double)tot_slack;
+        if (threadIdx.x == 0) atomicOr(err, 1u);  // the caller grows the buffers
+    } else if (tot_need + tot_slack > capb) {
+        alpha = (double)(capb - tot_need) / (double)tot_slack;
     }
-    // caps and per-thread sums
     unsigned long long local = 0;
     for (int t = t0; t < t1; t++) {
-        uint32_t n, nb = 0;
-        if (mode == 0) {
-            n = clamped_count(cnt_in, cap_in, t);
-            int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
-            if (tx > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - 1));
-            if (tx < g.ntx - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + 1));
-            if (ty > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx));
-            if (ty < g.nty - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx));
-            if (tz > 0) nb = max(nb, clamped_count(cnt_in, cap_in, t - g.ntx * g.nty));
-            if (tz < g.ntz - 1) nb = max(nb, clamped_count(cnt_in, cap_in, t + g.ntx * g.nty));
-        } else {
-            n = hist[t];
-        }
-        uint32_t s = mode == 0 ? max(n / 8, nb / 4) + 32 : n / 8 + 32;
-        uint32_t c = n + (uint32_t)(alpha * (double)s);
-        if (tot_need > (unsigned long long)cap_bins) c = 0;
-        cap_out[t] = c;
+        uint32_t n, s;
+        sizes(t, n, s);
+        uint32_t own = short_buf ? 0u : n;
+        uint32_t c = own + (uint32_t)(alpha * (double)s);
+        out.bin_cap[t] = c;
+        out.bin_off[t] = own;
         local += c;
     }
-    // exclusive scan of the per-thread sums (Hillis-Steele over 1024 entries)
     s_scan[threadIdx.x] = local;
     __syncthreads();
     for (int off = 1; off < PLAN_THREADS; off <<= 1) {
@@ -528,16 +520,12 @@ __global__ void __launch_bounds__(PLAN_THREADS)
     }
     unsigned long long run = s_scan[threadIdx.x] - local;
     for (int t = t0; t < t1; t++) {
-        start_out[t] = (uint32_t)run;
-        run += cap_out[t];
-        count_out[t] = 0;
+        out.bin_start[t] = (uint32_t)run;
+        run += out.bin_cap[t];
+        out.bin_own[t] = 0;
+        out.bin_in[t] = 0;
     }
-    if (threadIdx.x == 0) {
-        *stage_count_out = 0;
-        if (np_dev) *np_dev = 0;
-        if (diag)
-            for (int q = 0; q < NSLOT_DIAG; q++) diag[q] = 0;
-    }
+    if (threadIdx.x == 0) *out.stage_count = 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -552,16 +540,17 @@ struct PushOut {
     uint32_t *err;
 };
 
+// slot of an arrival in the inflow segment of tile dtile, or the staging list when full
 template <typename P>
-__device__ inline void append_particle(const Grid &g, const PartView &out, const P &p, int dtile,
-                                       uint32_t pos, const PushOut &O) {
+__device__ inline void store_inflow(const PartView &out, const P &p, int dtile, uint32_t pos, uint32_t *err) {
+    uint32_t off = out.bin_off[dtile], cap = out.bin_cap[dtile];
     long idx;
-    if (pos < out.bin_cap[dtile]) {
-        idx = (long)out.bin_start[dtile] + pos;
+    if (pos < cap - off) {
+        idx = (long)out.bin_start[dtile] + off + pos;
     } else {
         uint32_t s = atomicAdd(out.stage_count, 1u);
         if (s >= out.cap_stage) {
-            atomicOr(O.err, 2u);  // staging full: particle dropped, reported as capacity error
+            atomicOr(err, 2u);  // staging full: particle dropped, reported as a capacity error
             return;
         }
         idx = out.cap_bins + s;
@@ -576,31 +565,38 @@ __global__ void __launch_bounds__(PUSH_THREADS)
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
     using WT = typename std::conditional<PREC == PIC_MIXED, float, double>::type;
     const Grid &g = P.g;
-    __shared__ uint32_t sCnt[NDEST + 1], sBase[NDEST + 1];
-    __shared__ double sRed[NSLOT_DIAG + 1][PUSH_THREADS / 64];
+    __shared__ uint32_t sOwn;
+    __shared__ double sRed[NDIAG_SLOTS][PUSH_THREADS / 64];
     // LDS: E tile (float4 per node, or 3 double planes) and the deposit window
     __shared__ __attribute__((aligned(16))) unsigned char
         sRaw[(PREC == PIC_MIXED ? TNODES * 16 : TNODES * 24)];
     __shared__ WT sW[WNODES];
 
-    double acc[NSLOT_DIAG] = {0, 0, 0, 0, 0};
-    double alive_count = 0;
-
+    double acc[NDIAG_SLOTS] = {0, 0, 0, 0, 0, 0};
     const int tid = threadIdx.x;
+    auto tally = [&](const PT &p) {
+        double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
+        acc[0] += w;
+        acc[1] += vx * w;
+        acc[2] += vy * w;
+        acc[3] += vz * w;
+        acc[4] += w * (vx * vx + vy * vy + vz * vz);
+        acc[5] += 1;
+    };
+
     if ((int)blockIdx.x < g.ntiles) {
-        // ------------------------------------------------ binned tile
+        // ------------------------------------------------ binned tile: waves run independently
         int t = tile_of_block(blockIdx.x, g.ntiles, P.remap);
-        uint32_t cnt = clamped_count(in.bin_count, in.bin_cap, t);
+        BinSeg bs = bin_of(in, t);
+        uint32_t cnt = bs.c0 + bs.c1;
         if (cnt == 0) {
             if (tid == 0) O.win_used[t] = 0;
             if (tid < NDIAG_SLOTS) O.diag_part[tid * gridDim.x + blockIdx.x] = 0.0;
             return;
         }
-        uint32_t start = in.bin_start[t];
         int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
         int c0x = tx * TILE, c0y = ty * TILE, c0z = tz * TILE;
         long sj = g.ni, sk = (long)g.ni * g.nj;
-        // load the E tile
         for (int n = tid; n < TNODES; n += PUSH_THREADS) {
             int a = n % TN, b = (n / TN) % TN, c = n / (TN * TN);
             int gi = c0x + a, gj = c0y + b, gk = c0z + c;
@@ -618,113 +614,88 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             }
         }
         for (int n = tid; n < WNODES; n += PUSH_THREADS) sW[n] = WT(0);
-        if (tid <= NDEST) sCnt[tid] = 0;
+        if (tid == 0) sOwn = 0;
         __syncthreads();
+        const long own0 = out.bin_start[t];  // own segment of this tile in the output buffer
+        const uint32_t own_cap = out.bin_off[t];
 
-        for (uint32_t base = 0; base < cnt; base += PUSH_BATCH) {
-            PT pp[PUSH_PER_THREAD];
-            int dest[PUSH_PER_THREAD], dtile[PUSH_PER_THREAD];
-            uint32_t rank[PUSH_PER_THREAD];
-#pragma unroll
-            for (int r = 0; r < PUSH_PER_THREAD; r++) {
-                uint32_t q = base + r * PUSH_THREADS + tid;
-                dest[r] = -1;
-                dtile[r] = -1;
-                if (q < cnt) {
-                    PT &p = pp[r];
-                    load_p(in, (long)start + q, p);
-                    bool alive;
-                    int ni_, nj_, nk_;
-                    if constexpr (PREC == PIC_MIXED) {
-                        int li = min(max(p.ci - c0x, 0), TILE - 1);
-                        int lj = min(max(p.cj - c0y, 0), TILE - 1);
-                        int lk = min(max(p.ck - c0z, 0), TILE - 1);
-                        float3 E = gather_lds(reinterpret_cast<const float4 *>(sRaw), li, lj, lk,
-                                              p.ox, p.oy, p.oz);
-                        alive = push_mixed(P, p, E);
-                        ni_ = p.ci;
-                        nj_ = p.cj;
-                        nk_ = p.ck;
-                        if (alive) {
-                            int wi = p.ci - c0x + HALO, wj = p.cj - c0y + HALO, wk = p.ck - c0z + HALO;
-                            if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO &&
-                                wk >= 0 && wk < TILE + 2 * HALO)
-                                deposit_lds<float>(reinterpret_cast<float *>(sW), wi, wj, wk, p.ox,
-                                                   p.oy, p.oz, p.w);
-                            else
-                                deposit_global64(g, O.den_extra, p.ci, p.cj, p.ck, (double)p.ox,
-                                                 (double)p.oy, (double)p.oz, (double)p.w);
-                        }
-                    } else {
-                        int i, j, k;
-                        double di, dj, dk;
-                        cell_of(g, p, i, j, k, di, dj, dk);
-                        int li = min(max(i - c0x, 0), TILE - 1);
-                        int lj = min(max(j - c0y, 0), TILE - 1);
-                        int lk = min(max(k - c0z, 0), TILE - 1);
-                        const double *pl = reinterpret_cast<const double *>(sRaw);
-                        double3 E = gather64_weights(pl, pl + TNODES, pl + 2 * TNODES,
-                                                     li + TN * (lj + TN * lk), TN, TN * TN, di, dj, dk);
-                        alive = push_f64(P, p, E);
-                        if (alive) {
-                            cell_of(g, p, i, j, k, di, dj, dk);
-                            ni_ = i;
-                            nj_ = j;
-                            nk_ = k;
-                            int wi = i - c0x + HALO, wj = j - c0y + HALO, wk = k - c0z + HALO;
-                            if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO &&
-                                wk >= 0 && wk < TILE + 2 * HALO)
-                                deposit_lds<double>(reinterpret_cast<double *>(sW), wi, wj, wk, di, dj,
-                                                    dk, p.w);
-                            else
-                                deposit_global64(g, O.den_extra, i, j, k, di, dj, dk, p.w);
-                        }
-                    }
+        for (uint32_t base = 0; base < cnt; base += PUSH_THREADS) {
+            uint32_t q = base + tid;
+            PT p;
+            bool alive = false;
+            int dtile = -1;
+            if (q < cnt) {
+                load_p(in, seg_index(bs, q), p);
+                if constexpr (PREC == PIC_MIXED) {
+                    int li = min(max(p.ci - c0x, 0), TILE - 1);
+                    int lj = min(max(p.cj - c0y, 0), TILE - 1);
+                    int lk = min(max(p.ck - c0z, 0), TILE - 1);
+                    float3 E = gather_lds(reinterpret_cast<const float4 *>(sRaw), li, lj, lk, p.ox, p.oy, p.oz);
+                    alive = push_mixed(P, p, E);
                     if (alive) {
-                        double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
-                        acc[0] += w;
-                        acc[1] += vx * w;
-                        acc[2] += vy * w;
-                        acc[3] += vz * w;
-                        acc[4] += w * (vx * vx + vy * vy + vz * vz);
-                        alive_count += 1;
-                        int ntx_ = ni_ >> TILE_SHIFT, nty_ = nj_ >> TILE_SHIFT, ntz_ = nk_ >> TILE_SHIFT;
-                        int dx = ntx_ - tx, dy = nty_ - ty, dz = ntz_ - tz;
-                        dtile[r] = ntx_ + g.ntx * (nty_ + g.nty * ntz_);
-                        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
-                            dest[r] = (dx + 1) + 3 * ((dy + 1) + 3 * (dz + 1));
+                        int wi = p.ci - c0x + HALO, wj = p.cj - c0y + HALO, wk = p.ck - c0z + HALO;
+                        if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO && wk >= 0 &&
+                            wk < TILE + 2 * HALO)
+                            deposit_lds<float>(reinterpret_cast<float *>(sW), wi, wj, wk, p.ox, p.oy, p.oz, p.w);
                         else
-                            dest[r] = NDEST;  // far mover
+                            deposit_global64(g, O.den_extra, p.ci, p.cj, p.ck, (double)p.ox, (double)p.oy,
+                                             (double)p.oz, (double)p.w);
+                        dtile = tile_id(g, p.ci, p.cj, p.ck);
+                    }
+                } else {
+                    int i, j, k;
+                    double di, dj, dk;
+                    cell_of(g, p, i, j, k, di, dj, dk);
+                    int li = min(max(i - c0x, 0), TILE - 1);
+                    int lj = min(max(j - c0y, 0), TILE - 1);
+                    int lk = min(max(k - c0z, 0), TILE - 1);
+                    const double *pl = reinterpret_cast<const double *>(sRaw);
+                    double3 E = gather64_weights(pl, pl + TNODES, pl + 2 * TNODES, li + TN * (lj + TN * lk), TN,
+                                                 TN * TN, di, dj, dk);
+                    alive = push_f64(P, p, E);
+                    if (alive) {
+                        cell_of(g, p, i, j, k, di, dj, dk);
+                        int wi = i - c0x + HALO, wj = j - c0y + HALO, wk = k - c0z + HALO;
+                        if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO && wk >= 0 &&
+                            wk < TILE + 2 * HALO)
+                            deposit_lds<double>(reinterpret_cast<double *>(sW), wi, wj, wk, di, dj, dk, p.w);
+                        else
+                            deposit_global64(g, O.den_extra, i, j, k, di, dj, dk, p.w);
+                        dtile = tile_id(g, i, j, k);
+                    }
+                }
+                if (alive) tally(p);
+            }
+            // stayers: one LDS atomic per wavefront reserves their slots in the own segment
+            bool stay = alive && dtile == t;
+            unsigned long long m = __ballot(stay);
+            if (m) {
+                uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                int leader = __ffsll((long long)m) - 1;
+                uint32_t wbase = 0;
+                if ((int)__lane_id() == leader) wbase = atomicAdd(&sOwn, (uint32_t)__popcll(m));
+                wbase = __shfl(wbase, leader, 64);
+                if (stay) {
+                    uint32_t slot = wbase + before;
+                    if (slot < own_cap) {
+                        store_p(out, own0 + slot, p);
+                    } else {  // bins shrunk to nothing by a short buffer: spill to staging
+                        uint32_t st = atomicAdd(out.stage_count, 1u);
+                        if (st < out.cap_stage)
+                            store_p(out, out.cap_bins + st, p);
+                        else
+                            atomicOr(O.err, 2u);
                     }
                 }
             }
-            // reserve slots: wave-aggregated LDS counters per destination tile
-#pragma unroll
-            for (int r = 0; r < PUSH_PER_THREAD; r++) {
-                int key = (dest[r] >= 0 && dest[r] < NDEST) ? dest[r] : -1;
-                rank[r] = wave_reserve(key, sCnt);
-            }
-            __syncthreads();
-            if (tid < NDEST && sCnt[tid] > 0) {
-                int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
-                int dt = (tx + dx) + g.ntx * ((ty + dy) + g.nty * (tz + dz));
-                sBase[tid] = atomicAdd(&out.bin_count[dt], sCnt[tid]);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < PUSH_PER_THREAD; r++) {
-                if (dest[r] < 0) continue;
-                if (dest[r] < NDEST) {
-                    append_particle(g, out, pp[r], dtile[r], sBase[dest[r]] + rank[r], O);
-                } else {
-                    uint32_t pos = atomicAdd(&out.bin_count[dtile[r]], 1u);
-                    append_particle(g, out, pp[r], dtile[r], pos, O);
-                }
-            }
-            __syncthreads();
-            if (tid <= NDEST) sCnt[tid] = 0;
-            __syncthreads();
+            // arrivals elsewhere: wavefront-aggregated reservation in the destination's
+            // inflow segment
+            int key = (alive && !stay) ? dtile : -1;
+            uint32_t pos = wave_reserve_global(key, out.bin_in);
+            if (key >= 0) store_inflow(out, p, key, pos, O.err);
         }
+        __syncthreads();
         // flush the deposit window of this tile
         if (PREC == PIC_MIXED) {
             float *dst = O.win32 + (long)t * WNODES;
@@ -733,14 +704,16 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             double *dst = O.win64 + (long)t * WNODES;
             for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = reinterpret_cast<double *>(sW)[n];
         }
-        if (tid == 0) O.win_used[t] = 1;
+        if (tid == 0) {
+            O.win_used[t] = 1;
+            out.bin_own[t] = sOwn;
+        }
     } else {
         // ------------------------------------------------ unbinned staging list
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
         uint32_t nthreads = (gridDim.x - g.ntiles) * PUSH_THREADS;
         uint32_t first = (blockIdx.x - g.ntiles) * PUSH_THREADS + tid;
-        // uniform trip count per wavefront so that wave_reserve sees all lanes
-        uint32_t trips = (ns + nthreads - 1) / nthreads;
+        uint32_t trips = (ns + nthreads - 1) / nthreads;  // uniform per wavefront
         for (uint32_t it = 0; it < trips; it++) {
             uint32_t q = first + it * nthreads;
             PT p;
@@ -769,32 +742,19 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                         dt = tile_id(g, i, j, k);
                     }
                 }
-                if (alive) {
-                    double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
-                    acc[0] += w;
-                    acc[1] += vx * w;
-                    acc[2] += vy * w;
-                    acc[3] += vz * w;
-                    acc[4] += w * (vx * vx + vy * vy + vz * vz);
-                    alive_count += 1;
-                }
+                if (alive) tally(p);
             }
-            uint32_t pos = wave_reserve_global(dt, out.bin_count);
-            if (dt >= 0) append_particle(g, out, p, dt, pos, O);
+            uint32_t pos = wave_reserve_global(dt, out.bin_in);
+            if (dt >= 0) store_inflow(out, p, dt, pos, O.err);
         }
     }
-    // diagnostics of the pushed particles (getRealCount, getMomentum, getKE) and np
+    // per-block partials of getRealCount, getMomentum, getKE and the particle count
 #pragma unroll
-    for (int q = 0; q < NSLOT_DIAG; q++) {
+    for (int q = 0; q < NDIAG_SLOTS; q++) {
         double v = warp_sum(acc[q]);
         if ((tid & 63) == 0) sRed[q][tid >> 6] = v;
     }
-    {
-        double v = warp_sum(alive_count);
-        if ((tid & 63) == 0) sRed[NSLOT_DIAG][tid >> 6] = v;
-    }
     __syncthreads();
-    // one partial per block and slot: no two blocks write the same address
     if (tid < NDIAG_SLOTS) {
         double s = 0;
         for (int i = 0; i < PUSH_THREADS / 64; i++) s += sRed[tid][i];
@@ -841,48 +801,40 @@ __global__ void __launch_bounds__(256)
 template <int PREC>
 __global__ void __launch_bounds__(256) k_deposit_all(Grid g, PartView in, double *den_extra) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    auto dep = [&](const PT &p) {
+        if constexpr (PREC == PIC_MIXED) {
+            deposit_global64(g, den_extra, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz, p.w);
+        } else {
+            int i, j, k;
+            double di, dj, dk;
+            cell_of(g, p, i, j, k, di, dj, dk);
+            deposit_global64(g, den_extra, i, j, k, di, dj, dk, p.w);
+        }
+    };
     int b = blockIdx.x;
     if (b < g.ntiles) {
-        uint32_t cnt = clamped_count(in.bin_count, in.bin_cap, b);
-        uint32_t start = in.bin_start[b];
-        for (uint32_t q = threadIdx.x; q < cnt; q += blockDim.x) {
+        BinSeg bs = bin_of(in, b);
+        for (uint32_t q = threadIdx.x; q < bs.c0 + bs.c1; q += blockDim.x) {
             PT p;
-            load_p(in, (long)start + q, p);
-            if constexpr (PREC == PIC_MIXED)
-                deposit_global64(g, den_extra, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz, p.w);
-            else {
-                int i, j, k;
-                double di, dj, dk;
-                cell_of(g, p, i, j, k, di, dj, dk);
-                deposit_global64(g, den_extra, i, j, k, di, dj, dk, p.w);
-            }
+            load_p(in, seg_index(bs, q), p);
+            dep(p);
         }
     } else {
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
-        for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns;
-             q += (gridDim.x - g.ntiles) * blockDim.x) {
+        for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns; q += (gridDim.x - g.ntiles) * blockDim.x) {
             PT p;
             load_p(in, in.cap_bins + q, p);
-            if constexpr (PREC == PIC_MIXED)
-                deposit_global64(g, den_extra, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz, p.w);
-            else {
-                int i, j, k;
-                double di, dj, dk;
-                cell_of(g, p, i, j, k, di, dj, dk);
-                deposit_global64(g, den_extra, i, j, k, di, dj, dk, p.w);
-            }
+            dep(p);
         }
     }
 }
 
-// diagnostics over all particles (when the fused ones are stale)
+// diagnostics over all particles (when the fused ones are stale): per-block partials
 template <int PREC>
 __global__ void __launch_bounds__(256) k_diag_all(Grid g, PartView in, double *diag_part) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
     __shared__ double sh[8];
-    double acc[NSLOT_DIAG] = {0, 0, 0, 0, 0};
-    double cnt_all = 0;
-    int b = blockIdx.x;
+    double acc[NDIAG_SLOTS] = {0, 0, 0, 0, 0, 0};
     auto add = [&](const PT &p) {
         double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
         acc[0] += w;
@@ -890,31 +842,28 @@ __global__ void __launch_bounds__(256) k_diag_all(Grid g, PartView in, double *d
         acc[2] += vy * w;
         acc[3] += vz * w;
         acc[4] += w * (vx * vx + vy * vy + vz * vz);
-        cnt_all += 1;
+        acc[5] += 1;
     };
+    int b = blockIdx.x;
     if (b < g.ntiles) {
-        uint32_t cnt = clamped_count(in.bin_count, in.bin_cap, b);
-        uint32_t start = in.bin_start[b];
-        for (uint32_t q = threadIdx.x; q < cnt; q += blockDim.x) {
+        BinSeg bs = bin_of(in, b);
+        for (uint32_t q = threadIdx.x; q < bs.c0 + bs.c1; q += blockDim.x) {
             PT p;
-            load_p(in, (long)start + q, p);
+            load_p(in, seg_index(bs, q), p);
             add(p);
         }
     } else {
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
-        for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns;
-             q += (gridDim.x - g.ntiles) * blockDim.x) {
+        for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns; q += (gridDim.x - g.ntiles) * blockDim.x) {
             PT p;
             load_p(in, in.cap_bins + q, p);
             add(p);
         }
     }
-    for (int q = 0; q < NSLOT_DIAG; q++) {
+    for (int q = 0; q < NDIAG_SLOTS; q++) {
         double s = block_sum(acc[q], sh);
         if (threadIdx.x == 0) diag_part[q * gridDim.x + blockIdx.x] = s;
     }
-    double s = block_sum(cnt_all, sh);
-    if (threadIdx.x == 0) diag_part[NSLOT_DIAG * gridDim.x + blockIdx.x] = s;
 }
 
 // sums nslot rows of nb per-block partials in a fixed order (one workgroup)
@@ -931,32 +880,53 @@ __global__ void __launch_bounds__(1024) k_reduce_slots(const double *part, int n
 // ---------------------------------------------------------------------------------------
 // counting sort: histogram -> plan (mode 1) -> scatter
 // ---------------------------------------------------------------------------------------
+// a block's share of the particles: its tile's two segments, or a slice of the staging list
+struct Share {
+    BinSeg bs;
+    bool staging;
+    long sbase;
+    uint32_t cnt, first, step;
+};
+
+__device__ inline Share share_of(const Grid &g, const PartView &in) {
+    Share s{};
+    int b = blockIdx.x;
+    if (b < g.ntiles) {
+        s.bs = bin_of(in, b);
+        s.staging = false;
+        s.cnt = s.bs.c0 + s.bs.c1;
+        s.first = threadIdx.x;
+        s.step = blockDim.x;
+    } else {
+        s.staging = true;
+        s.sbase = in.cap_bins;
+        s.cnt = min(*in.stage_count, (uint32_t)in.cap_stage);
+        s.first = (b - g.ntiles) * blockDim.x + threadIdx.x;
+        s.step = (gridDim.x - g.ntiles) * blockDim.x;
+    }
+    return s;
+}
+
+__device__ inline long share_index(const Share &s, uint32_t q) {
+    return s.staging ? s.sbase + q : seg_index(s.bs, q);
+}
+
+__device__ inline uint32_t share_trips(const Share &s) {
+    uint32_t base = s.first - threadIdx.x;  // the same for every thread of the block
+    return s.cnt > base ? (s.cnt - base + s.step - 1) / s.step : 0;
+}
+
 template <int PREC>
 __global__ void __launch_bounds__(256) k_sort_hist(Grid g, PartView in, uint32_t *hist) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
-    int b = blockIdx.x;
-    uint32_t cnt, start;
-    long base;
-    uint32_t step, first;
-    if (b < g.ntiles) {
-        cnt = clamped_count(in.bin_count, in.bin_cap, b);
-        start = in.bin_start[b];
-        base = start;
-        first = threadIdx.x;
-        step = blockDim.x;
-    } else {
-        cnt = min(*in.stage_count, (uint32_t)in.cap_stage);
-        base = in.cap_bins;
-        first = (b - g.ntiles) * blockDim.x + threadIdx.x;
-        step = (gridDim.x - g.ntiles) * blockDim.x;
-    }
-    uint32_t trips = cnt > first - threadIdx.x ? (cnt - (first - threadIdx.x) + step - 1) / step : 0;
+    Share s = share_of(g, in);
+    uint32_t trips = share_trips(s);
     for (uint32_t it = 0; it < trips; it++) {
-        uint32_t q = first + it * step;
+        uint32_t q = s.first + it * s.step;
         int key = -1;
-        if (q < cnt) {
+        if (q < s.cnt) {
             PT p;
-            load_p(in, base + q, p);
+            load_p(in, share_index(s, q), p);
             key = tile_of(g, p);
         }
         wave_count_u32(key, hist);
@@ -966,36 +936,22 @@ __global__ void __launch_bounds__(256) k_sort_hist(Grid g, PartView in, uint32_t
 template <int PREC>
 __global__ void __launch_bounds__(256) k_sort_scatter(Grid g, PartView in, PartView out, uint32_t *err) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
-    int b = blockIdx.x;
-    uint32_t cnt;
-    long base;
-    uint32_t step, first;
-    if (b < g.ntiles) {
-        cnt = clamped_count(in.bin_count, in.bin_cap, b);
-        base = in.bin_start[b];
-        first = threadIdx.x;
-        step = blockDim.x;
-    } else {
-        cnt = min(*in.stage_count, (uint32_t)in.cap_stage);
-        base = in.cap_bins;
-        first = (b - g.ntiles) * blockDim.x + threadIdx.x;
-        step = (gridDim.x - g.ntiles) * blockDim.x;
-    }
-    uint32_t trips = cnt > first - threadIdx.x ? (cnt - (first - threadIdx.x) + step - 1) / step : 0;
+    Share s = share_of(g, in);
+    uint32_t trips = share_trips(s);
     for (uint32_t it = 0; it < trips; it++) {
-        uint32_t q = first + it * step;
+        uint32_t q = s.first + it * s.step;
         int key = -1;
         PT p;
-        if (q < cnt) {
-            load_p(in, base + q, p);
+        if (q < s.cnt) {
+            load_p(in, share_index(s, q), p);
             key = tile_of(g, p);
         }
-        uint32_t pos = wave_reserve_global(key, out.bin_count);
+        uint32_t pos = wave_reserve_global(key, out.bin_own);
         if (key >= 0) {
-            if (pos < out.bin_cap[key])
+            if (pos < out.bin_off[key])
                 store_p(out, (long)out.bin_start[key] + pos, p);
             else
-                atomicOr(err, 4u);  // cannot happen: caps >= histogram
+                atomicOr(err, 4u);  // cannot happen: own segments are sized by the histogram
         }
     }
 }
@@ -1075,26 +1031,11 @@ __global__ void __launch_bounds__(256)
     k_gather_out(Grid g, PartView in, const uint32_t *offsets, double *pos, double *vel, double *mpw,
                  uint32_t *ids) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
-    int b = blockIdx.x;
-    uint32_t cnt;
-    long base;
-    uint32_t step, first, o;
-    if (b < g.ntiles) {
-        cnt = clamped_count(in.bin_count, in.bin_cap, b);
-        base = in.bin_start[b];
-        first = threadIdx.x;
-        step = blockDim.x;
-        o = offsets[b];
-    } else {
-        cnt = min(*in.stage_count, (uint32_t)in.cap_stage);
-        base = in.cap_bins;
-        first = (b - g.ntiles) * blockDim.x + threadIdx.x;
-        step = (gridDim.x - g.ntiles) * blockDim.x;
-        o = offsets[g.ntiles];
-    }
-    for (uint32_t q = first; q < cnt; q += step) {
+    Share s = share_of(g, in);
+    uint32_t o = offsets[blockIdx.x < (unsigned)g.ntiles ? blockIdx.x : g.ntiles];
+    for (uint32_t q = s.first; q < s.cnt; q += s.step) {
         PT p;
-        load_p(in, base + q, p);
+        load_p(in, share_index(s, q), p);
         long d = (long)o + q;
         if constexpr (PREC == PIC_MIXED) {
             pos[3 * d + 0] = g.x0[0] + ((double)p.ci + (double)p.ox) * g.dh[0];
@@ -1266,7 +1207,7 @@ static int alloc_buf(pic_species *s, PartBuf &b, long cap_bins, long cap_stage) 
     size_t idb = s->track_ids ? (((size_t)n * 4 + 255) & ~(size_t)255) : 0;
     bytes += idb;
     size_t binb = (((size_t)g.ntiles * 4 + 255) & ~(size_t)255);
-    bytes += binb * 3 + 256;
+    bytes += binb * 5 + 256;
     void *mem = nullptr;
     PIC_HIP(hipMalloc(&mem, bytes));
     b = PartBuf{};
@@ -1299,11 +1240,13 @@ static int alloc_buf(pic_species *s, PartBuf &b, long cap_bins, long cap_stage) 
     }
     if (s->track_ids) b.id = (uint32_t *)take(idb);
     b.bin_start = (uint32_t *)take(binb);
-    b.bin_count = (uint32_t *)take(binb);
     b.bin_cap = (uint32_t *)take(binb);
+    b.bin_off = (uint32_t *)take(binb);
+    b.bin_own = (uint32_t *)take(binb);
+    b.bin_in = (uint32_t *)take(binb);
     b.stage_count = (uint32_t *)take(256);
     // empty bins, empty staging
-    PIC_HIP(hipMemsetAsync(b.bin_start, 0, binb * 3 + 256, s->w->stream));
+    PIC_HIP(hipMemsetAsync(b.bin_start, 0, binb * 5 + 256, s->w->stream));
     return PIC_OK;
 }
 
@@ -1317,9 +1260,7 @@ static long default_stage_cap(long cap_bins) { return std::max<long>(65536, cap_
 static int launch_plan_push(pic_species *s, uint32_t *err) {
     pic_world *w = s->w;
     PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
-    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(w->g, in.bin_count, in.bin_cap, nullptr, 0, out.bin_start,
-                                              out.bin_cap, out.bin_count, out.stage_count, out.cap_bins,
-                                              nullptr, nullptr, err);
+    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(w->g, view_of(in), nullptr, 0, view_of(out), err);
     PIC_LAUNCH_CHECK();
     return PIC_OK;
 }
@@ -1382,17 +1323,15 @@ static int sort_launch(pic_species *s) {
     PartView vin = view_of(in), vout = view_of(out);
     hipEvent_t ev;
     timer_begin(w, PIC_T_SORT, &ev);
-    // histogram into out.bin_count (zeroed), then plan caps/starts from it
-    PIC_HIP(hipMemsetAsync(out.bin_count, 0, sizeof(uint32_t) * g.ntiles, w->stream));
+    // histogram into out.bin_in (zeroed), kept in out.bin_cap through the plan, which then
+    // sizes the own segments from it and zeroes the counters
+    PIC_HIP(hipMemsetAsync(out.bin_in, 0, sizeof(uint32_t) * g.ntiles, w->stream));
     unsigned grid = g.ntiles + STAGE_WGS * 4;
-    k_sort_hist<PREC><<<grid, 256, 0, w->stream>>>(g, vin, out.bin_count);
+    k_sort_hist<PREC><<<grid, 256, 0, w->stream>>>(g, vin, out.bin_in);
     PIC_LAUNCH_CHECK();
-    // hist must survive the plan, which zeroes the counts: copy it to the cap array first
-    PIC_HIP(hipMemcpyAsync(out.bin_cap, out.bin_count, sizeof(uint32_t) * g.ntiles,
-                           hipMemcpyDeviceToDevice, w->stream));
-    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(g, nullptr, nullptr, out.bin_cap, 1, out.bin_start,
-                                              out.bin_cap, out.bin_count, out.stage_count, out.cap_bins,
-                                              nullptr, nullptr, err_flag(s));
+    PIC_HIP(hipMemcpyAsync(out.bin_cap, out.bin_in, sizeof(uint32_t) * g.ntiles, hipMemcpyDeviceToDevice,
+                           w->stream));
+    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(g, vin, out.bin_cap, 1, vout, err_flag(s));
     PIC_LAUNCH_CHECK();
     k_sort_scatter<PREC><<<grid, 256, 0, w->stream>>>(g, vin, vout, err_flag(s));
     PIC_LAUNCH_CHECK();
@@ -1411,13 +1350,20 @@ static int do_sort(pic_species *s) {
 static int count_particles(pic_species *s, std::vector<uint32_t> &cnt, uint32_t &stage) {
     const Grid &g = s->w->g;
     PartBuf &b = s->buf[s->cur];
-    std::vector<uint32_t> cap(g.ntiles);
-    cnt.resize(g.ntiles);
-    PIC_HIP(hipMemcpyAsync(cnt.data(), b.bin_count, 4 * (size_t)g.ntiles, hipMemcpyDeviceToHost, s->w->stream));
-    PIC_HIP(hipMemcpyAsync(cap.data(), b.bin_cap, 4 * (size_t)g.ntiles, hipMemcpyDeviceToHost, s->w->stream));
+    size_t nt = (size_t)g.ntiles;
+    std::vector<uint32_t> cap(nt), off(nt), own(nt), inn(nt);
+    cnt.resize(nt);
+    PIC_HIP(hipMemcpyAsync(cap.data(), b.bin_cap, 4 * nt, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipMemcpyAsync(off.data(), b.bin_off, 4 * nt, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipMemcpyAsync(own.data(), b.bin_own, 4 * nt, hipMemcpyDeviceToHost, s->w->stream));
+    PIC_HIP(hipMemcpyAsync(inn.data(), b.bin_in, 4 * nt, hipMemcpyDeviceToHost, s->w->stream));
     PIC_HIP(hipMemcpyAsync(&s->w->scal_host->iters, b.stage_count, 4, hipMemcpyDeviceToHost, s->w->stream));
     PIC_HIP(hipStreamSynchronize(s->w->stream));
-    for (int t = 0; t < g.ntiles; t++) cnt[t] = std::min(cnt[t], cap[t]);
+    for (size_t t = 0; t < nt; t++) {
+        uint32_t c0 = std::min(own[t], off[t]);
+        uint32_t room = cap[t] > off[t] ? cap[t] - off[t] : 0u;
+        cnt[t] = c0 + std::min(inn[t], room);
+    }
     stage = std::min<uint32_t>((uint32_t)s->w->scal_host->iters, (uint32_t)b.cap_stage);
     return PIC_OK;
 }
@@ -1590,8 +1536,10 @@ int pic_species_set_particles(pic_species *s, long n, const double *pos, const d
     // drop everything: empty bins and staging
     for (int b = 0; b < 2; b++) {
         PartBuf &pb = s->buf[b];
-        PIC_HIP(hipMemsetAsync(pb.bin_count, 0, 4 * (size_t)w->g.ntiles, w->stream));
+        PIC_HIP(hipMemsetAsync(pb.bin_own, 0, 4 * (size_t)w->g.ntiles, w->stream));
+        PIC_HIP(hipMemsetAsync(pb.bin_in, 0, 4 * (size_t)w->g.ntiles, w->stream));
         PIC_HIP(hipMemsetAsync(pb.bin_cap, 0, 4 * (size_t)w->g.ntiles, w->stream));
+        PIC_HIP(hipMemsetAsync(pb.bin_off, 0, 4 * (size_t)w->g.ntiles, w->stream));
         PIC_HIP(hipMemsetAsync(pb.stage_count, 0, 4, w->stream));
     }
     s->next_id = 0;

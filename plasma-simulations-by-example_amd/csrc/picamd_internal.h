@@ -74,8 +74,12 @@ struct PartBuf {
     double *x = nullptr, *y = nullptr, *z = nullptr;
     double *dvx = nullptr, *dvy = nullptr, *dvz = nullptr, *dw = nullptr;
     uint32_t *id = nullptr;
-    // tile bins
-    uint32_t *bin_start = nullptr, *bin_count = nullptr, *bin_cap = nullptr;
+    // tile bins.  Bin t holds two segments: [start, start + own) and
+    // [start + off, start + off + min(in, cap - off)).  A push writes the particles that stay
+    // in tile t to the first (sized to t's input count, so it never overflows) and appends
+    // arrivals from other tiles to the second.
+    uint32_t *bin_start = nullptr, *bin_cap = nullptr, *bin_off = nullptr;
+    uint32_t *bin_own = nullptr, *bin_in = nullptr;
     uint32_t *stage_count = nullptr;  // device counter of the staging list
 };
 
@@ -85,14 +89,33 @@ struct PartView {
     float *ox, *oy, *oz, *vx, *vy, *vz, *w;
     double *x, *y, *z, *dvx, *dvy, *dvz, *dw;
     uint32_t *id;
-    uint32_t *bin_start, *bin_count, *bin_cap, *stage_count;
+    uint32_t *bin_start, *bin_cap, *bin_off, *bin_own, *bin_in, *stage_count;
     long cap_bins, cap_stage;
 };
 
 inline PartView view_of(const PartBuf &b) {
     return PartView{b.cell, b.ox, b.oy, b.oz, b.vx, b.vy, b.vz, b.w, b.x, b.y, b.z,
-                    b.dvx, b.dvy, b.dvz, b.dw, b.id, b.bin_start, b.bin_count, b.bin_cap,
-                    b.stage_count, b.cap_bins, b.cap_stage};
+                    b.dvx, b.dvy, b.dvz, b.dw, b.id, b.bin_start, b.bin_cap, b.bin_off,
+                    b.bin_own, b.bin_in, b.stage_count, b.cap_bins, b.cap_stage};
+}
+
+// the two segments of tile t: q in [0, c0+c1) lives at q < c0 ? s0+q : s1+(q-c0)
+struct BinSeg {
+    uint32_t s0, c0, s1, c1;
+};
+
+__device__ inline BinSeg bin_of(const PartView &v, int t) {
+    uint32_t cap = v.bin_cap[t], off = v.bin_off[t], st = v.bin_start[t];
+    uint32_t c0 = v.bin_own[t];
+    c0 = c0 < off ? c0 : off;
+    uint32_t room = cap > off ? cap - off : 0u;
+    uint32_t c1 = v.bin_in[t];
+    c1 = c1 < room ? c1 : room;
+    return BinSeg{st, c0, st + off, c1};
+}
+
+__device__ inline long seg_index(const BinSeg &b, uint32_t q) {
+    return q < b.c0 ? (long)b.s0 + q : (long)b.s1 + (q - b.c0);
 }
 
 // ---- device scalars of a world ---------------------------------------------------------
@@ -168,7 +191,8 @@ struct pic_solver {
     double tol = 1e-4;
     double phi0 = 0, n0 = 1e12, Te0 = 1.5;  // defaults of ch3/ver2/PotentialSolver.h:93-95
     // NR-PCG vectors
-    double *y = nullptr, *g = nullptr, *d = nullptr, *z = nullptr, *F = nullptr, *P = nullptr;
+    // (P holds the Jacobian diagonal jd = A(u,u) - P(u); d and z2 are the ping-pong pair of d)
+    double *y = nullptr, *g = nullptr, *d = nullptr, *z = nullptr, *F = nullptr, *P = nullptr, *z2 = nullptr;
 };
 
 // ---- error handling ------------------------------------------------------------------

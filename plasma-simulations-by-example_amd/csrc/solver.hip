@@ -6,7 +6,9 @@
 // stopping test, a different iterate path.  Converged solutions agree to the tolerance.
 // All arithmetic is FP64; dots and norms are FP64 sums with a fixed reduction tree.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "picamd_internal.h"
 
@@ -282,17 +284,18 @@ __device__ inline double row_apply(const PcgParams &Q, int type, long u, int i, 
     return s;
 }
 
-// F = A x - b, minus the Boltzmann term on regular nodes; P = d(ne term)/dphi
-// (ch3/ver2/PotentialSolver.cpp:240-264)
+// F = A x - b, minus the Boltzmann term on regular nodes, and the Jacobian diagonal
+// jd = A(u,u) - P(u), P = d(ne term)/dphi (ch3/ver2/PotentialSolver.cpp:240-266)
 __global__ void __launch_bounds__(256)
-    k_nr_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *P) {
+    k_nr_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *jd) {
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
     int i, j, k;
     node_ijk(g, u, i, j, k);
     int t = node_type(g, oid, u, i, j, k);
-    double Ax = row_apply(Q, t, u, i, j, k, a_diag(Q, t, i, j), x);
+    double ad = a_diag(Q, t, i, j);
+    double Ax = row_apply(Q, t, u, i, j, k, ad, x);
     double b;
     if (t == NT_NEU) b = 0;
     else if (t == NT_DIR) b = x[u];
@@ -304,17 +307,22 @@ __global__ void __launch_bounds__(256)
         p = Q.n0 * QE / (EPS_0 * Q.Te0) * exp((x[u] - Q.phi0) / Q.Te0);
     }
     F[u] = f;
-    P[u] = p;
+    jd[u] = ad - p;  // Matrix::diagSubtract (:60-64)
 }
 
-__device__ inline double j_diag(const PcgParams &Q, int t, int i, int j, const double *P, long u) {
-    return a_diag(Q, t, i, j) - P[u];
-}
+// ---- PCG (solvePCGLinear, :299-331) in two vector passes per iteration -----------------
+// Iteration k of the reference:  z = J d; a = g.s; b = d.z; y += (a/b) d; g += (a/b) z;
+// s = M g; a' = g.s; d = (a'/a) d - s; stop when RMS(g) < tol.  Here:
+//   pass A:  d_new = mu d_old - M g (recomputed at the 7 stencil points), y += lam d_old
+//            (the previous iteration's deferred update), z = J d_new, partials d_new.z
+//   pass B:  g += (a/b) z, partials g.(M g) and g.g
+// with M g = (1.0/jd) g as Matrix::invDiagonal.  d ping-pongs between two buffers so that
+// pass A reads only old values.  The last deferred y update is applied by k_pcg_finish.
 
-// PCG init: g = J y - F, s = M g, d = -s; partials of g.s
+// g = J y - F; d_0 = 0; partials of g.s  (s = M g)
 __global__ void __launch_bounds__(256)
-    k_pcg_init(PcgParams Q, const uint8_t *oid, const double *P, const double *y, const double *F, double *gv,
-               double *d, double *partials) {
+    k_pcg_init(PcgParams Q, const uint8_t *oid, const double *jd, const double *y, const double *F, double *gv,
+               double *d0, double *partials) {
     __shared__ double sh[4];
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
@@ -323,57 +331,81 @@ __global__ void __launch_bounds__(256)
         int i, j, k;
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
-        double jd = j_diag(Q, t, i, j, P, u);
-        double gg = row_apply(Q, t, u, i, j, k, jd, y) - F[u];
-        double M = 1.0 / jd;
-        double s = M * gg;
+        double gg = row_apply(Q, t, u, i, j, k, jd[u], y) - F[u];
+        double s = (1.0 / jd[u]) * gg;
         gv[u] = gg;
-        d[u] = -1 * s;
+        d0[u] = 0;
         gs = gg * s;
     }
     double r = block_sum(gs, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
-// z = J d; partials of d.z
+__device__ inline double dnew_at(const double *d_old, const double *gv, const double *jd, long v, double mu) {
+    return mu * d_old[v] - (1.0 / jd[v]) * gv[v];
+}
+
 __global__ void __launch_bounds__(256)
-    k_pcg_spmv(PcgParams Q, const uint8_t *oid, const double *P, const double *d, double *z, double *partials,
-               const DevScalars *sc) {
+    k_pcg_a(PcgParams Q, const uint8_t *oid, const double *jd, const double *gv, const double *d_old,
+            double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
+    double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
     if (u < g.nn) {
         int i, j, k;
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
-        double zz = row_apply(Q, t, u, i, j, k, j_diag(Q, t, i, j, P, u), d);
+        long sj = g.ni, sk = (long)g.ni * g.nj;
+        y[u] = y[u] + lam * d_old[u];
+        double dc = dnew_at(d_old, gv, jd, u, mu);
+        double zz;
+        if (t == NT_DIR) {
+            zz = jd[u] * dc;
+        } else if (t == NT_REG) {
+            zz = 0;
+            zz += Q.idz2 * dnew_at(d_old, gv, jd, u - sk, mu);
+            zz += Q.idy2 * dnew_at(d_old, gv, jd, u - sj, mu);
+            zz += Q.idx2 * dnew_at(d_old, gv, jd, u - 1, mu);
+            zz += jd[u] * dc;
+            zz += Q.idx2 * dnew_at(d_old, gv, jd, u + 1, mu);
+            zz += Q.idy2 * dnew_at(d_old, gv, jd, u + sj, mu);
+            zz += Q.idz2 * dnew_at(d_old, gv, jd, u + sk, mu);
+        } else {
+            long nb;
+            double a;
+            if (i == 0) { nb = u + 1; a = -Q.idx; }
+            else if (i == g.ni - 1) { nb = u - 1; a = -Q.idx; }
+            else if (j == 0) { nb = u + sj; a = -Q.idy; }
+            else if (j == g.nj - 1) { nb = u - sj; a = -Q.idy; }
+            else if (k == 0) { nb = u + sk; a = -Q.idz; }
+            else { nb = u - sk; a = -Q.idz; }
+            zz = 0;
+            zz += jd[u] * dc;
+            zz += a * dnew_at(d_old, gv, jd, nb, mu);
+        }
+        d_new[u] = dc;
         z[u] = zz;
-        dz = d[u] * zz;
+        dz = dc * zz;
     }
     double r = block_sum(dz, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
-// y += l d; g += l z (l = alpha/beta); s = M g; partials of g.s and g.g
 __global__ void __launch_bounds__(256)
-    k_pcg_update(PcgParams Q, const uint8_t *oid, const double *P, double *y, double *gv, const double *d,
-                 const double *z, double *partials, int nb, const DevScalars *sc) {
+    k_pcg_b(PcgParams Q, const double *jd, double *gv, const double *z, double *partials, int nb,
+            const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const Grid &g = Q.g;
     double alpha = sc->red[0], beta = sc->red[1];
     double gs = 0, g2 = 0;
-    if (u < g.nn) {
-        int i, j, k;
-        node_ijk(g, u, i, j, k);
-        int t = node_type(g, oid, u, i, j, k);
-        y[u] = y[u] + (alpha / beta) * d[u];
+    if (u < Q.g.nn) {
         double gg = gv[u] + (alpha / beta) * z[u];
         gv[u] = gg;
-        double s = (1.0 / j_diag(Q, t, i, j, P, u)) * gg;
+        double s = (1.0 / jd[u]) * gg;
         gs = gg * s;
         g2 = gg * gg;
     }
@@ -385,49 +417,38 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// d = (alpha_new/alpha) d - s
-__global__ void __launch_bounds__(256)
-    k_pcg_dir(PcgParams Q, const uint8_t *oid, const double *P, const double *gv, double *d, const DevScalars *sc) {
-    if (sc->done & 2) return;
+// the deferred y update of the last completed iteration
+__global__ void __launch_bounds__(256) k_pcg_finish(long nn, const double *d, double *y, const DevScalars *sc) {
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const Grid &g = Q.g;
-    if (u >= g.nn) return;
-    int i, j, k;
-    node_ijk(g, u, i, j, k);
-    int t = node_type(g, oid, u, i, j, k);
-    double s = (1.0 / j_diag(Q, t, i, j, P, u)) * gv[u];
-    double alpha_new = sc->red[3], alpha = sc->red[0];
-    d[u] = (alpha_new / alpha) * d[u] - s;
+    double lam = sc->red[7];
+    if (u < nn) y[u] = y[u] + lam * d[u];
 }
 
-// scalar steps between the vector kernels (one workgroup each)
-// mode 0: alpha = alpha_new = sum (init)
-// mode 1: beta = sum (after spmv)
-// mode 2: alpha_new = sum0, gg = sum1, convergence test (after update)
-// mode 3: alpha = alpha_new (after dir)
+// scalar steps (one workgroup): red[0] alpha, red[1] beta, red[2] g.g, red[3] alpha',
+// red[4] RMS(g), red[6] mu for the next pass A, red[7] lam (pending y update)
+// mode 0: after init;  mode 1: after pass A;  mode 2: after pass B (convergence test)
 __global__ void __launch_bounds__(1024)
     k_pcg_scalar(const double *partials, int nb, int mode, double nu, double tol, int it, DevScalars *sc) {
     __shared__ double sh[16];
-    if (mode != 0 && (sc->done & 1)) {
-        if (mode == 2 && threadIdx.x == 0) sc->done |= 2;  // converged earlier: freeze d
-        return;
-    }
-    if (mode == 3) {
-        if (threadIdx.x == 0) sc->red[0] = sc->red[3];
-        return;
-    }
+    if (mode != 0 && sc->done) return;
     double s0 = fixed_sum(partials, nb, sh);
     double s1 = mode == 2 ? fixed_sum(partials + nb, nb, sh) : 0.0;
     if (threadIdx.x != 0) return;
     if (mode == 0) {
         sc->red[0] = s0;
-        sc->red[3] = s0;
+        sc->red[6] = 0.0;  // d_0 = -s_0
+        sc->red[7] = 0.0;
+        sc->iters = 0;
     } else if (mode == 1) {
         sc->red[1] = s0;
     } else {
+        double alpha = sc->red[0], beta = sc->red[1];
+        sc->red[7] = alpha / beta;  // y += (alpha/beta) d, applied by the next pass A
         sc->red[3] = s0;
         sc->red[2] = s1;
-        double l2 = sqrt(s1 / nu);  // vec::norm is RMS (ch3/ver2/PotentialSolver.cpp:114-121)
+        sc->red[6] = s0 / alpha;   // d = (alpha'/alpha) d - s
+        sc->red[0] = s0;
+        double l2 = sqrt(s1 / nu);  // vec::norm is RMS (:114-121)
         sc->red[4] = l2;
         sc->iters = it + 1;
         if (l2 < tol) sc->done = 1;
@@ -436,7 +457,7 @@ __global__ void __launch_bounds__(1024)
 
 // red-black Gauss-Seidel (w = 1) on J y = F: the fallback solveGSLinear (:433-461)
 __global__ void __launch_bounds__(256)
-    k_gsl_sweep(PcgParams Q, const uint8_t *oid, const double *P, double *y, const double *F, int color,
+    k_gsl_sweep(PcgParams Q, const uint8_t *oid, const double *jdv, double *y, const double *F, int color,
                 const DevScalars *sc) {
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -446,14 +467,14 @@ __global__ void __launch_bounds__(256)
     node_ijk(g, u, i, j, k);
     if (((i + j + k) & 1) != color) return;
     int t = node_type(g, oid, u, i, j, k);
-    double jd = j_diag(Q, t, i, j, P, u);
+    double jd = jdv[u];
     double S = row_apply(Q, t, u, i, j, k, jd, y) - jd * y[u];
     double phi_new = (F[u] - S) / jd;
     y[u] = y[u] + 1. * (phi_new - y[u]);
 }
 
 __global__ void __launch_bounds__(256)
-    k_gsl_residual(PcgParams Q, const uint8_t *oid, const double *P, const double *y, const double *F,
+    k_gsl_residual(PcgParams Q, const uint8_t *oid, const double *jdv, const double *y, const double *F,
                    double *partials, const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
@@ -464,7 +485,7 @@ __global__ void __launch_bounds__(256)
         int i, j, k;
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
-        double R = row_apply(Q, t, u, i, j, k, j_diag(Q, t, i, j, P, u), y) - F[u];
+        double R = row_apply(Q, t, u, i, j, k, jdv[u], y) - F[u];
         r2 = R * R;
     }
     double r = block_sum(r2, sh);
@@ -617,31 +638,34 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     int rc = grow_partials(w, (int)nb);
     if (rc) return rc;
     double nu = (double)(int)nn;
+    double *dbuf[2] = {s->d, s->z2};
     k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
-    k_pcg_init<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, s->g, s->d, w->partials);
+    k_pcg_init<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, s->g, dbuf[0], w->partials);
     k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 0, nu, Q.tol, 0, w->scal);
     const int CHUNK = 32;
     bool conv = false;
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_pcg_spmv<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->d, s->z, w->partials, w->scal);
+            k_pcg_a<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+                                              s->z, w->partials, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 1, nu, Q.tol, it, w->scal);
-            k_pcg_update<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->g, s->d, s->z, w->partials,
-                                                   (int)nb, w->scal);
+            k_pcg_b<<<nb, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)nb, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 2, nu, Q.tol, it, w->scal);
-            k_pcg_dir<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, s->d, w->scal);
-            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 3, nu, Q.tol, it, w->scal);
         }
         PIC_LAUNCH_CHECK();
         rc = read_scalars(w);
         if (rc) return rc;
-        if (w->scal_host->done & 1) {
+        if (w->scal_host->done) {
             conv = true;
             break;
         }
     }
-    *its += conv ? w->scal_host->iters : s->max_it;
+    // d of the last completed iteration sits in dbuf[iters & 1]
+    int done_its = w->scal_host->iters;
+    k_pcg_finish<<<nb, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    PIC_LAUNCH_CHECK();
+    *its += done_its;
     *converged = conv;
     return PIC_OK;
 }
@@ -673,7 +697,7 @@ static int gs_linear(pic_solver *s, const PcgParams &Q) {
 static int ensure_pcg_vectors(pic_solver *s) {
     if (s->y) return PIC_OK;
     long nn = s->w->g.nn;
-    double **v[6] = {&s->y, &s->g, &s->d, &s->z, &s->F, &s->P};
+    double **v[7] = {&s->y, &s->g, &s->d, &s->z, &s->F, &s->P, &s->z2};
     for (auto p : v) PIC_HIP(hipMalloc(p, sizeof(double) * nn));
     return PIC_OK;
 }
@@ -689,9 +713,12 @@ static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
     if (rc) return rc;
     // y starts at zero for each solve and is warm-started across Newton steps (:232)
     k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
-    k_fill<<<nb, 256, 0, w->stream>>>(nn, s->P, 0.0);
     const int NR_MAX_IT = 20;
     const double NR_TOL = 1e-3;
+    // PIC_VERBOSE=1: one progress line per Newton iteration on stderr
+    const char *venv = getenv("PIC_VERBOSE");
+    bool verbose = venv && venv[0] == '1';
+    auto t0 = std::chrono::steady_clock::now();
     bool converged = false;
     double norm = 0;
     int pcg_its = 0;
@@ -711,6 +738,10 @@ static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
         rc = read_scalars(w);
         if (rc) return rc;
         norm = sqrt(w->scal_host->red[5] / (double)(int)nn);
+        if (verbose)
+            fprintf(stderr, "[picamd] NR it %d: PCG %s after %d its total, |y|rms %.3e, %.1f ms\n", it,
+                    pc ? "converged" : "fell back to GS", pcg_its, norm,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         if (norm < NR_TOL) {
             converged = true;
             break;
@@ -751,6 +782,7 @@ int pic_solver_destroy(pic_solver *s) {
     hipFree(s->z);
     hipFree(s->F);
     hipFree(s->P);
+    hipFree(s->z2);
     delete s;
     return PIC_OK;
 }

@@ -10,6 +10,7 @@ import atexit
 import ctypes as C
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -153,6 +154,9 @@ class World:
 
     def __init__(self, ni, nj, nk):
         self.h = _vp()
+        # species and solvers of this world: closed before it, whatever order the garbage
+        # collector finalizes them in
+        self._children = weakref.WeakSet()
         _check(lib().pic_world_create(ni, nj, nk, C.byref(self.h)))
         self.ni, self.nj, self.nk = ni, nj, nk
         self.nn = ni * nj * nk
@@ -160,6 +164,8 @@ class World:
 
     def close(self):
         if self.h and not _exiting:
+            for c in list(self._children):
+                c.close()
             lib().pic_world_destroy(self.h)
             self.h = _vp()
 
@@ -233,11 +239,12 @@ class Species:
         self.h = _vp()
         _check(lib().pic_species_create(world.h, mass, charge, mpw0, precision, int(capacity),
                                         1 if track_ids else 0, C.byref(self.h)))
+        world._children.add(self)
 
     def close(self):
-        if self.h and not _exiting:
+        if self.h and not _exiting and self.world.h:
             lib().pic_species_destroy(self.h)
-            self.h = _vp()
+        self.h = _vp()
 
     def __del__(self):
         try:
@@ -328,11 +335,12 @@ class PotentialSolver:
         self.world = world
         self.h = _vp()
         _check(lib().pic_solver_create(world.h, stype, max_it, tol, C.byref(self.h)))
+        world._children.add(self)
 
     def close(self):
-        if self.h and not _exiting:
+        if self.h and not _exiting and self.world.h:
             lib().pic_solver_destroy(self.h)
-            self.h = _vp()
+        self.h = _vp()
 
     def __del__(self):
         try:

@@ -3,7 +3,7 @@
 Tolerances: FP64 species and all FP64 node kernels must agree with the oracle to rounding
 (computeEF and node volumes bit-exact; deposits to 1e-13 relative, since atomic summation
 order differs).  PIC_MIXED species store the position as cell + FP32 offset and push in
-FP32: positions within 2e-4 cells (RMS 2e-6 cells), velocities within 2e-5 of the velocity
+FP32: positions within 2e-4 cells (RMS 1e-5 cells), velocities within 2e-5 of the velocity
 scale, densities within 1e-5 relative L2 over a 10-step horizon -- far inside the 1e-4
 relative field-error budget.
 """
@@ -159,7 +159,7 @@ def test_push_reflect_mixed_matches_oracle(vmax, dt):
     assert np.array_equal(r["gi"], r["ri"])
     dcell = np.abs(r["gp"] - r["rp"]) / g.dh
     assert dcell.max() < 2e-4, dcell.max()
-    assert np.sqrt(np.mean(dcell ** 2)) < 2e-6
+    assert np.sqrt(np.mean(dcell ** 2)) < 1e-5
     # FP32 velocities: errors are relative to the velocity scale, not to each component
     vscale = np.abs(r["rv"]).max()
     assert np.abs(r["gv"] - r["rv"]).max() < 2e-5 * vscale
@@ -190,7 +190,7 @@ def test_push_absorb_sphere_mixed_matches_oracle():
     rm = np.isin(r["ri"], common)
     dcell = np.abs(r["gp"][gm] - r["rp"][rm]) / r["g"].dh
     assert dcell.max() < 2e-4, dcell.max()
-    assert np.sqrt(np.mean(dcell ** 2)) < 2e-6
+    assert np.sqrt(np.mean(dcell ** 2)) < 1e-5
     assert rel_l2(r["den"], r["den_ref"]) < 1e-4
 
 

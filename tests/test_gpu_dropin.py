@@ -2,8 +2,9 @@
 and the ch3 sphere) and agree with the oracle replaying the same seeded run.
 
 With the solver tolerance tight (1e-8 here) the red-black and lexicographic SOR iterates
-coincide to ~1e-9, so the FP64 run matches the oracle to 1e-7 relative in every diagnostic
-and the mixed-precision run to 1e-4.
+coincide to ~1e-9, so the FP64 run matches the oracle to the 6 significant digits that
+runtime_diags.csv carries (the reference's default stream precision) and the
+mixed-precision run to 1e-4.
 """
 import csv
 import os
@@ -59,7 +60,7 @@ def oracle_config1(num_ts, tol):
     return out
 
 
-@pytest.mark.parametrize("fp64,rtol", [(1, 1e-7), (0, 1e-4)])
+@pytest.mark.parametrize("fp64,rtol", [(1, 5e-6), (0, 1e-4)])
 def test_ch2_box_dropin_matches_oracle(tmp_path, fp64, rtol):
     num_ts = 10
     stdout, rows = run_example(tmp_path, "ch2_box", num_ts, 1, fp64, 1e-8)
@@ -72,7 +73,8 @@ def test_ch2_box_dropin_matches_oracle(tmp_path, fp64, rtol):
     np.testing.assert_allclose(float(last["KE.e-"]), ref[ts]["KE_e"], rtol=rtol)
     np.testing.assert_allclose(float(last["KE.O+"]), ref[ts]["KE_i"], rtol=rtol)
     np.testing.assert_allclose(float(last["PE"]), ref[ts]["PE"], rtol=rtol)
-    assert os.path.exists(tmp_path / "results" / f"fields_{num_ts:05d}.vti")
+    # written after the loop, at ts = num_ts + 1 (advanceTime pre-increments)
+    assert os.path.exists(tmp_path / "results" / f"fields_{num_ts + 1:05d}.vti")
 
 
 def test_ch3_sphere_dropin_runs_and_conserves(tmp_path):
