@@ -48,11 +48,17 @@ extern "C" {
 #define PIC_BC_REFLECT 0 /* specular walls, ch2/Species.cpp:32-36 */
 #define PIC_BC_ABSORB 1  /* absorbing walls and sphere, ch3/ver2/Species.cpp:28-32 */
 
-/* solver types: GS, PCG, QN as ch3/ver2/PotentialSolver.h:40; SOR_LINEAR is the ch2 solver */
+/* solver types: GS, PCG, QN as ch3/ver2/PotentialSolver.h:40; SOR_LINEAR is the ch2 solver.
+ * PCG_SYM is the Newton + Jacobi-PCG of PCG on the system with the Neumann face nodes
+ * eliminated (each face node equals the interior node its copy chain reaches, as the
+ * reference's Neumann rows require at convergence): the same discrete solution, but a
+ * symmetric negative-definite Jacobian, so CG converges where the reference's one-sided
+ * Neumann rows (ch3/ver2/PotentialSolver.cpp:180-187) make it stall on large grids. */
 #define PIC_SOLVER_GS 0
 #define PIC_SOLVER_PCG 1
 #define PIC_SOLVER_QN 2
 #define PIC_SOLVER_SOR_LINEAR 3
+#define PIC_SOLVER_PCG_SYM 4
 
 /* world node fields for pic_world_get_field / pic_world_set_field */
 #define PIC_FIELD_PHI 0       /* World::phi */

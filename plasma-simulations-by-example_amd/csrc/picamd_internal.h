@@ -193,6 +193,7 @@ struct pic_solver {
     // NR-PCG vectors
     // (P holds the Jacobian diagonal jd = A(u,u) - P(u); d and z2 are the ping-pong pair of d)
     double *y = nullptr, *g = nullptr, *d = nullptr, *z = nullptr, *F = nullptr, *P = nullptr, *z2 = nullptr;
+    uint8_t *mask = nullptr;  // PCG_SYM: regular-neighbour bits per node
 };
 
 // ---- error handling ------------------------------------------------------------------

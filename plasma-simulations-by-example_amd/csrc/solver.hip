@@ -528,6 +528,212 @@ __global__ void k_fill(long n, double *p, double v) {
     if (u < n) p[u] = v;
 }
 
+// ---- Newton + PCG on the Neumann-reduced system (PIC_SOLVER_PCG_SYM) -------------------
+// Face nodes are not unknowns: each equals the node its copy chain reaches (the priority
+// i, j, k of ch3/ver2/PotentialSolver.cpp:359-370, which is what the reference's Neumann
+// rows enforce at convergence).  For a regular node every Neumann neighbour reaches the
+// node itself, so eliminating them adds their coefficient to the diagonal: the reduced
+// Jacobian is symmetric and negative definite.  Vectors are zero off the regular nodes.
+__device__ inline long chain_target(const Grid &g, const uint8_t *oid, long u, int i, int j, int k) {
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    for (int hop = 0; hop < 3; hop++) {
+        if (oid[u] > 0) return u;
+        if (i == 0) { i = 1; u += 1; }
+        else if (i == g.ni - 1) { i = g.ni - 2; u -= 1; }
+        else if (j == 0) { j = 1; u += sj; }
+        else if (j == g.nj - 1) { j = g.nj - 2; u -= sj; }
+        else if (k == 0) { k = 1; u += sk; }
+        else if (k == g.nk - 1) { k = g.nk - 2; u -= sk; }
+        else return u;
+    }
+    return u;
+}
+
+// direction d: 0 -z, 1 -y, 2 -x, 3 +x, 4 +y, 5 +z (the reference's slot order around the centre)
+__device__ inline long nb_offset(const Grid &g, int d) {
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    const long o[6] = {-sk, -sj, -1, 1, sj, sk};
+    return o[d];
+}
+
+__device__ inline double nb_coef(const PcgParams &Q, int d) {
+    const double c[6] = {Q.idz2, Q.idy2, Q.idx2, Q.idx2, Q.idy2, Q.idz2};
+    return c[d];
+}
+
+// x on face nodes := x at their chain target
+__global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid, double *x) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u >= g.nn) return;
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
+    if (node_type(g, oid, u, i, j, k) != NT_NEU) return;
+    x[u] = x[chain_target(g, oid, u, i, j, k)];
+}
+
+// F on regular nodes (the reference's residual, ch3/ver2/PotentialSolver.cpp:240-257), the
+// reduced diagonal jr = A(u,u) - P(u) + sum of Neumann-neighbour coefficients, and a mask:
+// bits 0-5 regular neighbours in direction d, bit 6 the node itself is regular.
+__global__ void __launch_bounds__(256)
+    k_nrs_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *jr,
+            uint8_t *mask) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    if (u >= g.nn) return;
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
+    int t = node_type(g, oid, u, i, j, k);
+    if (t != NT_REG) {
+        F[u] = 0;
+        jr[u] = 1;
+        mask[u] = 0;
+        return;
+    }
+    double ad = a_diag(Q, t, i, j);
+    double Ax = row_apply(Q, t, u, i, j, k, ad, x);
+    double f = Ax - (-rho[u] / EPS_0);
+    f -= QE * Q.n0 * exp((x[u] - Q.phi0) / Q.Te0) / EPS_0;
+    double p = Q.n0 * QE / (EPS_0 * Q.Te0) * exp((x[u] - Q.phi0) / Q.Te0);
+    double diag = ad - p;
+    uint8_t m = 1u << 6;
+    const int di[6] = {0, 0, -1, 1, 0, 0}, dj[6] = {0, -1, 0, 0, 1, 0}, dk[6] = {-1, 0, 0, 0, 0, 1};
+    for (int d = 0; d < 6; d++) {
+        long v = u + nb_offset(g, d);
+        int tv = node_type(g, oid, v, i + di[d], j + dj[d], k + dk[d]);
+        if (tv == NT_REG) m |= (uint8_t)(1u << d);
+        else if (tv == NT_NEU) diag += nb_coef(Q, d);
+    }
+    F[u] = f;
+    jr[u] = diag;
+    mask[u] = m;
+}
+
+__device__ inline double jr_apply(const PcgParams &Q, long u, uint8_t m, double jru, const double *v) {
+    const Grid &g = Q.g;
+    double s = 0;
+    for (int d = 0; d < 3; d++)
+        if (m & (1u << d)) s += nb_coef(Q, d) * v[u + nb_offset(g, d)];
+    s += jru * v[u];
+    for (int d = 3; d < 6; d++)
+        if (m & (1u << d)) s += nb_coef(Q, d) * v[u + nb_offset(g, d)];
+    return s;
+}
+
+__global__ void __launch_bounds__(256)
+    k_pcgs_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
+                double *d0, double *partials) {
+    __shared__ double sh[4];
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double gs = 0;
+    if (u < Q.g.nn) {
+        uint8_t m = mask[u];
+        double gg = 0;
+        if (m & 64) {
+            gg = jr_apply(Q, u, m, jr[u], y) - F[u];
+            gs = gg * ((1.0 / jr[u]) * gg);
+        }
+        gv[u] = gg;
+        d0[u] = 0;
+    }
+    double r = block_sum(gs, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(256)
+    k_pcgs_a(PcgParams Q, const uint8_t *mask, const double *jr, const double *gv, const double *d_old,
+             double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    double mu = sc->red[6], lam = sc->red[7];
+    double dz = 0;
+    if (u < g.nn) {
+        uint8_t m = mask[u];
+        if (m & 64) {
+            y[u] = y[u] + lam * d_old[u];
+            double dc = dnew_at(d_old, gv, jr, u, mu);
+            double zz = 0;
+            for (int d = 0; d < 3; d++)
+                if (m & (1u << d)) {
+                    long v = u + nb_offset(g, d);
+                    zz += nb_coef(Q, d) * dnew_at(d_old, gv, jr, v, mu);
+                }
+            zz += jr[u] * dc;
+            for (int d = 3; d < 6; d++)
+                if (m & (1u << d)) {
+                    long v = u + nb_offset(g, d);
+                    zz += nb_coef(Q, d) * dnew_at(d_old, gv, jr, v, mu);
+                }
+            d_new[u] = dc;
+            z[u] = zz;
+            dz = dc * zz;
+        } else {
+            d_new[u] = 0;
+            z[u] = 0;
+        }
+    }
+    double r = block_sum(dz, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// red-black Gauss-Seidel (w = 1) on the reduced system: the solveGSLinear fallback
+__global__ void __launch_bounds__(256)
+    k_gsls_sweep(PcgParams Q, const uint8_t *mask, const double *jr, double *y, const double *F, int color,
+                 const DevScalars *sc) {
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    if (u >= g.nn) return;
+    uint8_t m = mask[u];
+    if (!(m & 64)) return;
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
+    if (((i + j + k) & 1) != color) return;
+    double S = jr_apply(Q, u, m, jr[u], y) - jr[u] * y[u];
+    double phi_new = (F[u] - S) / jr[u];
+    y[u] = y[u] + 1. * (phi_new - y[u]);
+}
+
+__global__ void __launch_bounds__(256)
+    k_gsls_residual(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F,
+                    double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double r2 = 0;
+    if (u < Q.g.nn) {
+        uint8_t m = mask[u];
+        if (m & 64) {
+            double R = jr_apply(Q, u, m, jr[u], y) - F[u];
+            r2 = R * R;
+        }
+    }
+    double r = block_sum(r2, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// y on face nodes := y at their chain target, 0 on Dirichlet nodes; x -= y; partials of y.y
+__global__ void __launch_bounds__(256)
+    k_nrs_update(PcgParams Q, const uint8_t *oid, double *x, double *y, double *partials) {
+    __shared__ double sh[4];
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const Grid &g = Q.g;
+    double y2 = 0;
+    if (u < g.nn) {
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
+        int t = node_type(g, oid, u, i, j, k);
+        double yy = y[u];
+        if (t == NT_DIR) yy = 0;
+        else if (t == NT_NEU) yy = y[chain_target(g, oid, u, i, j, k)];
+        x[u] = x[u] - yy;
+        y2 = yy * yy;
+    }
+    double r = block_sum(y2, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
 static unsigned nbk(long n) { return (unsigned)((n + 255) / 256); }
 
 static SorParams sor_params(const pic_solver *s) {
@@ -753,6 +959,118 @@ static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
     return converged ? PIC_OK : PIC_NOT_CONVERGED;
 }
 
+static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *its) {
+    pic_world *w = s->w;
+    long nn = Q.g.nn;
+    unsigned nb = nbk(nn);
+    double nu = (double)(int)nn;
+    double *dbuf[2] = {s->d, s->z2};
+    k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
+    k_pcgs_init<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0], w->partials);
+    k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 0, nu, Q.tol, 0, w->scal);
+    const int CHUNK = 32;
+    bool conv = false;
+    for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
+        int it1 = std::min(s->max_it, it0 + CHUNK);
+        for (int it = it0; it < it1; it++) {
+            k_pcgs_a<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+                                               s->z, w->partials, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 1, nu, Q.tol, it, w->scal);
+            k_pcg_b<<<nb, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)nb, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 2, nu, Q.tol, it, w->scal);
+        }
+        PIC_LAUNCH_CHECK();
+        int rc = read_scalars(w);
+        if (rc) return rc;
+        if (w->scal_host->done) {
+            conv = true;
+            break;
+        }
+    }
+    int done_its = w->scal_host->iters;
+    k_pcg_finish<<<nb, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    PIC_LAUNCH_CHECK();
+    *its += done_its;
+    *converged = conv;
+    return PIC_OK;
+}
+
+static int gsls_linear(pic_solver *s, const PcgParams &Q) {
+    pic_world *w = s->w;
+    long nn = Q.g.nn;
+    unsigned nb = nbk(nn);
+    k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
+    const int CHUNK = 100;
+    for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
+        int it1 = std::min(s->max_it, it0 + CHUNK);
+        for (int it = it0; it < it1; it++) {
+            k_gsls_sweep<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->y, s->F, 0, w->scal);
+            k_gsls_sweep<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->y, s->F, 1, w->scal);
+            if (it % 25 == 0) {
+                k_gsls_residual<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->y, s->F, w->partials, w->scal);
+                k_sor_check<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, (double)(int)nn, Q.tol, it, w->scal);
+            }
+        }
+        PIC_LAUNCH_CHECK();
+        int rc = read_scalars(w);
+        if (rc) return rc;
+        if (w->scal_host->done) break;
+    }
+    return PIC_OK;
+}
+
+// Newton iteration of solveNRPCG (:225-296) on the Neumann-reduced system
+static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
+    pic_world *w = s->w;
+    int rc = ensure_pcg_vectors(s);
+    if (rc) return rc;
+    if (!s->mask) PIC_HIP(hipMalloc(&s->mask, w->g.nn));
+    PcgParams Q = pcg_params(s);
+    long nn = w->g.nn;
+    unsigned nb = nbk(nn);
+    rc = grow_partials(w, (int)nb);
+    if (rc) return rc;
+    const char *venv = getenv("PIC_VERBOSE");
+    bool verbose = venv && venv[0] == '1';
+    auto t0 = std::chrono::steady_clock::now();
+    k_neumann_copy<<<nb, 256, 0, w->stream>>>(w->g, w->object_id, w->phi);
+    k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
+    const int NR_MAX_IT = 20;
+    const double NR_TOL = 1e-3;
+    bool converged = false;
+    double norm = 0;
+    int pcg_its = 0;
+    for (int it = 0; it < NR_MAX_IT; it++) {
+        k_nrs_F<<<nb, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask);
+        PIC_LAUNCH_CHECK();
+        bool pc = false;
+        rc = pcgs_linear(s, Q, &pc, &pcg_its);
+        if (rc) return rc;
+        if (!pc) {
+            rc = gsls_linear(s, Q);
+            if (rc) return rc;
+        }
+        k_nrs_update<<<nb, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
+        k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, &w->scal->red[5]);
+        PIC_LAUNCH_CHECK();
+        rc = read_scalars(w);
+        if (rc) return rc;
+        norm = sqrt(w->scal_host->red[5] / (double)(int)nn);
+        if (verbose)
+            fprintf(stderr, "[picamd] NR(sym) it %d: PCG %s after %d its total, |y|rms %.3e, %.1f ms\n", it,
+                    pc ? "converged" : "fell back to GS", pcg_its, norm,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        if (norm < NR_TOL) {
+            converged = true;
+            break;
+        }
+    }
+    if (residual) *residual = norm;
+    if (iters) *iters = pcg_its;
+    w->ef_valid = false;
+    return converged ? PIC_OK : PIC_NOT_CONVERGED;
+}
+
 }  // namespace pic
 
 using namespace pic;
@@ -761,7 +1079,7 @@ extern "C" {
 
 int pic_solver_create(pic_world *w, int type, int max_it, double tol, pic_solver **out) {
     PIC_CHECK(w && out, PIC_ERR_ARG, "null argument");
-    PIC_CHECK(type >= PIC_SOLVER_GS && type <= PIC_SOLVER_SOR_LINEAR, PIC_ERR_ARG, "bad solver type");
+    PIC_CHECK(type >= PIC_SOLVER_GS && type <= PIC_SOLVER_PCG_SYM, PIC_ERR_ARG, "bad solver type");
     PIC_CHECK(max_it >= 0, PIC_ERR_ARG, "negative max_it");
     PIC_CHECK(w->extents_set, PIC_ERR_STATE, "setExtents must precede the solver");
     pic_solver *s = new pic_solver();
@@ -783,6 +1101,7 @@ int pic_solver_destroy(pic_solver *s) {
     hipFree(s->F);
     hipFree(s->P);
     hipFree(s->z2);
+    hipFree(s->mask);
     delete s;
     return PIC_OK;
 }
@@ -806,6 +1125,7 @@ int pic_solver_solve(pic_solver *s, double *residual, int *iters) {
     case PIC_SOLVER_SOR_LINEAR: rc = solve_sor<false>(s, residual, iters); break;
     case PIC_SOLVER_GS: rc = solve_sor<true>(s, residual, iters); break;
     case PIC_SOLVER_PCG: rc = solve_nrpcg(s, residual, iters); break;
+    case PIC_SOLVER_PCG_SYM: rc = solve_nrpcg_sym(s, residual, iters); break;
     default:
         rc = solve_qn(w, s->phi0, s->Te0, s->n0);
         if (residual) *residual = 0;

@@ -8,7 +8,8 @@
 using namespace std;
 
 PotentialSolver::PotentialSolver(World &world, SolverType type, int max_it, double tol)
-    : world(world), kind(type == GS ? PIC_SOLVER_GS : type == PCG ? PIC_SOLVER_PCG : PIC_SOLVER_QN) {
+    : world(world),
+      kind(type == GS ? PIC_SOLVER_GS : type == PCG ? PIC_SOLVER_PCG : type == PCGS ? PIC_SOLVER_PCG_SYM : PIC_SOLVER_QN) {
     pic_check(pic_solver_create(world.dev(), kind, max_it, tol, &handle), "pic_solver_create");
     buildMatrix();
 }
@@ -31,7 +32,7 @@ void PotentialSolver::buildMatrix() { pic_check(pic_solver_solve_qn(handle), "so
 bool PotentialSolver::solve() {
     int rc = pic_solver_solve(handle, &last_res, &last_its);
     if (rc == PIC_NOT_CONVERGED) {
-        if (kind == PIC_SOLVER_PCG)
+        if (kind == PIC_SOLVER_PCG || kind == PIC_SOLVER_PCG_SYM)
             cout << "NR+PCG failed to converge, norm = " << last_res << endl;
         else
             cerr << "GS failed to converge, L2=" << last_res << endl;

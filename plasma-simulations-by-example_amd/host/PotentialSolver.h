@@ -8,7 +8,8 @@
 
 #include "World.h"
 
-enum SolverType { GS, PCG, QN };
+// PCGS: Newton + PCG on the Neumann-reduced (symmetric) Jacobian, PIC_SOLVER_PCG_SYM
+enum SolverType { GS, PCG, QN, PCGS };
 
 class PotentialSolver {
 public:

@@ -259,3 +259,20 @@ def test_nrpcg_matches_oracle():
     # same algorithm; only the dot-product summation order differs
     assert abs(its - st[1]) <= 2, (its, st)
     assert rel_l2(w.field(P.PHI), ref) < 1e-8
+
+
+def test_nrpcg_sym_same_solution_as_reference_nrpcg():
+    """PIC_SOLVER_PCG_SYM (Neumann nodes eliminated) converges to the solution the
+    reference's NR-PCG converges to; both stop on RMS(y) < 1e-3 V."""
+    w, g, oid, phi, rho = _sphere_world()
+    O.solve_qn(g, phi, g.zeros(), oid, 0, 1.5, 1e12)
+    w.set_field(P.PHI, phi)
+    w.set_field(P.RHO, rho)
+    s = P.PotentialSolver(w, P.PCG_SYM, 10000, 1e-8)
+    s.set_reference_values(0, 1.5, 1e10)
+    ok, res, its = s.solve()
+    assert ok and res < 1e-3
+    ref = phi.copy()
+    ok_r, st = O.solve_nrpcg(g, ref, rho, oid, 0, 1.5, 1e10, 10000, 1e-8)
+    assert ok_r
+    np.testing.assert_allclose(w.field(P.PHI), ref, rtol=0, atol=2e-3)
