@@ -50,7 +50,45 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--T-ion", type=float, default=1000.0)
+    ap.add_argument("--push-only", action="store_true",
+                    help="profiling mode: the same particle workload, pushes only (no solve), "
+                         "plus a 1 GiB device copy that calibrates the HBM byte counters")
     return ap.parse_args()
+
+
+def push_only(P, args, grid):
+    """Loads the workload's particles and runs `steps` advance+computeNumberDensity steps in
+    the field of the sheath-free potential (phi = 0 with the sphere at -100 V), printing
+    the push kernel's average duration.  Used under rocprofv3 --pmc."""
+    import torch
+    ni, nj, nk = grid
+    w = P.World(ni, nj, nk)
+    w.set_extents(Case.x0, Case.xm)
+    w.add_sphere(Case.sphere_c, Case.sphere_r, Case.phi_s)
+    w.add_inlet()
+    vol = np.prod(np.subtract(Case.xm, Case.x0))
+    ions = P.Species(w, 16 * AMU, QE, Case.n0 * vol / args.particles, P.MIXED,
+                     capacity=int(args.particles * 1.02))
+    ions.load_box_device(Case.x0, Case.xm, Case.n0, int(args.particles), Case.v_drift, args.T_ion,
+                         seed=1)
+    s = P.PotentialSolver(w, P.QN, 1, 1e-4)
+    s.compute_ef()
+    w.set_timing(True)
+    for i in range(args.warmup + args.steps):
+        if i == args.warmup:
+            w.timers(reset=True)
+        ions.advance(P.ABSORB, Case.dt)
+        ions.compute_number_density()
+    tm = w.timers(reset=True)
+    n = ions.np()
+    # calibration: a plain copy of 1 GiB (reads 1 GiB, writes 1 GiB)
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"push_only": True, "particles": n, "push_avg_ms": tm["push"][0] / tm["push"][1],
+                      "deposit_avg_ms": tm["deposit"][0] / tm["deposit"][1],
+                      "copy_bytes_each_way": 1 << 30}), flush=True)
 
 
 def log(*a):
@@ -216,6 +254,9 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     grid = tuple(int(x) for x in args.grid.split(","))
+    if args.push_only:
+        push_only(P, args, grid)
+        return
     mixed = run_gpu(P, grid, args.particles, P.MIXED, args.steps, args.warmup, args.T_ion,
                     want_fields=not args.no_cpu)
     log(f"[bench] mixed: {mixed['particle_steps_per_s']:.4e} particle-steps/s, "
