@@ -62,7 +62,6 @@ def push_only(P, args, grid):
     """Loads the workload's particles and runs `steps` advance+computeNumberDensity steps in
     the field of the sheath-free potential (phi = 0 with the sphere at -100 V), printing
     the push kernel's average duration.  Used under rocprofv3 --pmc."""
-    import torch
     ni, nj, nk = grid
     w = P.World(ni, nj, nk)
     w.set_extents(Case.x0, Case.xm)
@@ -86,15 +85,16 @@ def push_only(P, args, grid):
         log(f"[bench] push-only {args.precision} step {i} done")
     tm = w.timers(reset=True)
     n = ions.np()
-    # calibration: a plain elementwise copy kernel over 1 GiB (reads 1 GiB, writes 1 GiB).
-    # b.copy_(a) would go to a DMA engine and show no kernel dispatch to the counters.
-    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.mul(a, 1.0, out=b)
-    torch.cuda.synchronize()
+    # calibration of the HBM byte counters: Field_::updateAverage (k_avg) is a plain
+    # streaming kernel over the nodes that reads den and den_ave (16 B/node) and writes
+    # den_ave (8 B/node).  Torch kernels are avoided here: launching one under a counter
+    # collection pass crashed the profiler's dispatch interception.
+    ions.update_averages()
+    w.sync()
     print(json.dumps({"push_only": True, "precision": args.precision, "particles": n, "push_avg_ms": tm["push"][0] / tm["push"][1],
                       "deposit_avg_ms": tm["deposit"][0] / tm["deposit"][1],
-                      "copy_bytes_each_way": 1 << 30}), flush=True)
+                      "calibration": {"kernel": "k_avg", "read_bytes": 16 * ni * nj * nk,
+                                      "write_bytes": 8 * ni * nj * nk}}), flush=True)
 
 
 def log(*a):
@@ -152,6 +152,7 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
         initial_potential(P, w)
     else:
         w.set_field(P.PHI, phi)
+    phi_init = w.field(P.PHI)
     # Newton + Jacobi-PCG with the reference's tolerances, on the Neumann-reduced system:
     # the reference's one-sided Neumann rows leave a non-symmetric Jacobian on which CG
     # stalls at this size (PIC_SOLVER_PCG_SYM reaches the same discrete solution)
@@ -203,9 +204,11 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     )
     out["particle_steps_per_s"] = np_sum / ((push_ms + dep_ms) * 1e-3)
     out["push_avg_ms"] = push_ms / max(push_calls, 1)
-    out["phi"] = w.field(P.PHI)
+    out["phi_init"] = phi_init
     if want_fields:
+        out["phi"] = w.field(P.PHI)
         out["ef"] = w.field(P.EF)
+        out["den"] = ions.den()
     ions.close()
     solver.close()
     w.close()
@@ -264,13 +267,15 @@ def main():
         push_only(P, args, grid)
         return
     mixed = run_gpu(P, grid, args.particles, P.MIXED, args.steps, args.warmup, args.T_ion,
-                    want_fields=not args.no_cpu)
+                    want_fields=True)
     log(f"[bench] mixed: {mixed['particle_steps_per_s']:.4e} particle-steps/s, "
         f"push {mixed['push_avg_ms']:.3f} ms/launch, {mixed['ms_per_step']:.1f} ms/step")
     fp64 = None
     if not args.no_fp64:
+        # the same seeded particles and the same initial potential: the field differences
+        # after warmup+steps steps are those of the particle arithmetic alone
         fp64 = run_gpu(P, grid, args.particles, P.FP64, args.steps, args.warmup, args.T_ion,
-                       phi=mixed["phi"])
+                       want_fields=True, phi=mixed["phi_init"])
         log(f"[bench] fp64: {fp64['particle_steps_per_s']:.4e} particle-steps/s, "
             f"push {fp64['push_avg_ms']:.3f} ms/launch")
 
@@ -338,6 +343,13 @@ def main():
                         "poisson_ms_per_step": fp64["solve_ms"] / steps,
                         "push_avg_ms": fp64["push_avg_ms"],
                         "mixed_speedup": mixed["particle_steps_per_s"] / fp64["particle_steps_per_s"]}
+
+        def rel_l2(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        line["mixed_vs_fp64_rel_l2"] = {
+            "den": rel_l2(mixed["den"], fp64["den"]), "phi": rel_l2(mixed["phi"], fp64["phi"]),
+            "ef": rel_l2(mixed["ef"], fp64["ef"]), "after_steps": args.warmup + steps,
+            "start": "same seeded particles, same initial phi"}
     print(json.dumps(line), flush=True)
 
 

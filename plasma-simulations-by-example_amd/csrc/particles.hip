@@ -55,44 +55,40 @@ __device__ inline int tile_of_block(int b, int ntiles, int remap) {
     return (b & 7) * per + (b >> 3);
 }
 
-// Ranks lanes that share a key within the wavefront (keys >= 0), reserving space for each
-// distinct key once per wavefront with one LDS atomic.  Returns the rank of this lane in
-// the block-wide counter of its key.
-__device__ inline uint32_t wave_reserve(int key, uint32_t *counter) {
-    uint32_t rank = 0;
+// Groups the lanes of a wavefront by key (keys >= 0) with ballots only: for each lane, the
+// lowest lane holding its key (its leader), its rank among those lanes and their number.
+struct KeyGroup {
+    int leader;
+    uint32_t before, count;
+};
+
+__device__ inline KeyGroup wave_group(int key) {
+    KeyGroup kg{-1, 0u, 0u};
     unsigned long long active = __ballot(key >= 0);
     while (active) {
         int leader = __ffsll((long long)active) - 1;
         int lk = __shfl(key, leader, 64);
         unsigned long long m = __ballot(key == lk);
-        uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-        uint32_t base = 0;
-        if ((int)__lane_id() == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (key == lk) rank = base + before;
+        if (key == lk) {
+            kg.leader = leader;
+            kg.before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            kg.count = (uint32_t)__popcll(m);
+        }
         active &= ~m;
     }
-    return rank;
+    return kg;
 }
 
-// Same for global per-tile counters (far movers, staging pushes, sort).
+// Reserves counter[key] slots for the lanes of each distinct key: the leaders of all keys
+// issue their atomics in one instruction (one round trip per wavefront, not one per key),
+// then each lane reads its leader's base.  Returns the lane's slot (0 for key < 0).
 __device__ inline uint32_t wave_reserve_global(int key, uint32_t *counter) {
-    uint32_t rank = 0;
-    unsigned long long active = __ballot(key >= 0);
-    while (active) {
-        int leader = __ffsll((long long)active) - 1;
-        int lk = __shfl(key, leader, 64);
-        unsigned long long m = __ballot(key == lk);
-        uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-        uint32_t base = 0;
-        if ((int)__lane_id() == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (key == lk) rank = base + before;
-        active &= ~m;
-    }
-    return rank;
+    KeyGroup kg = wave_group(key);
+    uint32_t base = 0;
+    if (key >= 0 && kg.before == 0) base = atomicAdd(&counter[key], kg.count);
+    int src = kg.leader < 0 ? (int)__lane_id() : kg.leader;
+    base = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)base);
+    return key >= 0 ? base + kg.before : 0u;
 }
 
 __device__ inline unsigned long long wave_count_u32(int key, uint32_t *counter) {
@@ -251,24 +247,22 @@ __device__ inline void reflect_axis(int &c, float &o, float &v, int nc) {
     }
 }
 
-// E at a particle from an LDS E tile (float4 per node, TN^3 nodes), trilinear (CIC).
-__device__ inline float3 gather_lds(const float4 *sE, int li, int lj, int lk, float ox, float oy,
-                                    float oz) {
+// E at a particle from the LDS E tile: three FP32 planes (Ex, Ey, Ez) of TN^3 nodes,
+// trilinear (CIC).  Planes instead of one float4 per node: 12 B/node of LDS, which lets
+// eight push workgroups share a CU.
+__device__ inline float3 gather_lds(const float *pl, int li, int lj, int lk, float ox, float oy, float oz) {
     int b = li + TN * (lj + TN * lk);
-    float4 e000 = sE[b], e100 = sE[b + 1], e010 = sE[b + TN], e110 = sE[b + TN + 1];
-    float4 e001 = sE[b + TN * TN], e101 = sE[b + TN * TN + 1], e011 = sE[b + TN * TN + TN],
-           e111 = sE[b + TN * TN + TN + 1];
     float ax = 1.0f - ox, ay = 1.0f - oy, az = 1.0f - oz;
     float w000 = ax * ay * az, w100 = ox * ay * az, w010 = ax * oy * az, w110 = ox * oy * az;
     float w001 = ax * ay * oz, w101 = ox * ay * oz, w011 = ax * oy * oz, w111 = ox * oy * oz;
-    float3 E;
-    E.x = e000.x * w000 + e100.x * w100 + e110.x * w110 + e010.x * w010 + e001.x * w001 +
-          e101.x * w101 + e111.x * w111 + e011.x * w011;
-    E.y = e000.y * w000 + e100.y * w100 + e110.y * w110 + e010.y * w010 + e001.y * w001 +
-          e101.y * w101 + e111.y * w111 + e011.y * w011;
-    E.z = e000.z * w000 + e100.z * w100 + e110.z * w110 + e010.z * w010 + e001.z * w001 +
-          e101.z * w101 + e111.z * w111 + e011.z * w011;
-    return E;
+    float r[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float *e = pl + c * TNODES + b;
+        r[c] = e[0] * w000 + e[1] * w100 + e[TN + 1] * w110 + e[TN] * w010 + e[TN * TN] * w001 +
+               e[TN * TN + 1] * w101 + e[TN * TN + TN + 1] * w111 + e[TN * TN + TN] * w011;
+    }
+    return make_float3(r[0], r[1], r[2]);
 }
 
 __device__ inline float3 gather_global32(const Grid &g, const float4 *ef, int ci, int cj, int ck,
@@ -544,22 +538,26 @@ struct PushOut {
     uint32_t *err;
 };
 
+// slot of an arrival in the inflow segment of tile dtile, -1 when the segment is full
+__device__ inline long inflow_slot(const PartView &out, int dtile, uint32_t pos) {
+    uint32_t off = out.bin_off[dtile], cap = out.bin_cap[dtile];
+    return pos < cap - off ? (long)out.bin_start[dtile] + off + pos : -1L;
+}
+
+// slot in the staging list, -1 (and a capacity error) when it is full
+__device__ inline long staging_slot(const PartView &out, uint32_t *err) {
+    uint32_t s = atomicAdd(out.stage_count, 1u);
+    if (s < out.cap_stage) return out.cap_bins + s;
+    atomicOr(err, 2u);  // staging full: particle dropped, reported as a capacity error
+    return -1L;
+}
+
 // slot of an arrival in the inflow segment of tile dtile, or the staging list when full
 template <typename P>
 __device__ inline void store_inflow(const PartView &out, const P &p, int dtile, uint32_t pos, uint32_t *err) {
-    uint32_t off = out.bin_off[dtile], cap = out.bin_cap[dtile];
-    long idx;
-    if (pos < cap - off) {
-        idx = (long)out.bin_start[dtile] + off + pos;
-    } else {
-        uint32_t s = atomicAdd(out.stage_count, 1u);
-        if (s >= out.cap_stage) {
-            atomicOr(err, 2u);  // staging full: particle dropped, reported as a capacity error
-            return;
-        }
-        idx = out.cap_bins + s;
-    }
-    store_p(out, idx, p);
+    long idx = inflow_slot(out, dtile, pos);
+    if (idx < 0) idx = staging_slot(out, err);
+    if (idx >= 0) store_p(out, idx, p);
 }
 
 template <int PREC>
@@ -570,15 +568,20 @@ __global__ void __launch_bounds__(PUSH_THREADS)
     const Grid &g = P.g;
     __shared__ uint32_t sOwn;
     __shared__ double sRed[NDIAG_SLOTS][PUSH_THREADS / 64];
-    // LDS: E tile (float4 per node, or 3 double planes) and the FP64 deposit window
+    // LDS: E tile (3 float or 3 double planes), the FP64 deposit window, and the output bin
+    // segments of the 27 tiles around this one (start, inflow offset, capacity)
     __shared__ __attribute__((aligned(16))) unsigned char
-        sRaw[(PREC == PIC_MIXED ? TNODES * 16 : TNODES * 24)];
+        sRaw[(PREC == PIC_MIXED ? TNODES * 12 : TNODES * 24)];
     __shared__ double sW[WNODES];
+    __shared__ uint32_t sNb[3][27];
 
-    double acc[NDIAG_SLOTS] = {0, 0, 0, 0, 0, 0};
+    // per-thread diagnostic sums in the particle arithmetic (a thread sees a few dozen
+    // particles of one tile), reduced in FP64 across the workgroup and the grid
+    using AT = typename std::conditional<PREC == PIC_MIXED, float, double>::type;
+    AT acc[NDIAG_SLOTS] = {0, 0, 0, 0, 0, 0};
     const int tid = threadIdx.x;
     auto tally = [&](const PT &p) {
-        double vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
+        AT vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
         acc[0] += w;
         acc[1] += vx * w;
         acc[2] += vy * w;
@@ -607,7 +610,10 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             long u = gi + sj * gj + sk * gk;
             if (PREC == PIC_MIXED) {
                 float4 e = ok ? ef32[u] : make_float4(0.f, 0.f, 0.f, 0.f);
-                reinterpret_cast<float4 *>(sRaw)[n] = e;
+                float *pl = reinterpret_cast<float *>(sRaw);
+                pl[n] = e.x;
+                pl[TNODES + n] = e.y;
+                pl[2 * TNODES + n] = e.z;
             } else {
                 double4 e = ok ? ef64[u] : make_double4(0, 0, 0, 0);
                 double *pl = reinterpret_cast<double *>(sRaw);
@@ -617,24 +623,39 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             }
         }
         for (int n = tid; n < WNODES; n += PUSH_THREADS) sW[n] = 0.0;
+        if (tid < 27) {
+            int nx = tx + tid % 3 - 1, ny = ty + (tid / 3) % 3 - 1, nz = tz + tid / 9 - 1;
+            bool in = nx >= 0 && nx < g.ntx && ny >= 0 && ny < g.nty && nz >= 0 && nz < g.ntz;
+            int nt = nx + g.ntx * (ny + g.nty * nz);
+            sNb[0][tid] = in ? out.bin_start[nt] : 0u;
+            sNb[1][tid] = in ? out.bin_off[nt] : 0u;
+            sNb[2][tid] = in ? out.bin_cap[nt] : 0u;
+        }
         if (tid == 0) sOwn = 0;
         __syncthreads();
         const long own0 = out.bin_start[t];  // own segment of this tile in the output buffer
         const uint32_t own_cap = out.bin_off[t];
 
+        // the next particle of each thread is loaded while the current one is processed
+        PT pn;
+        if (tid < cnt) load_p(in, seg_index(bs, tid), pn);
         for (uint32_t base = 0; base < cnt; base += PUSH_THREADS) {
             uint32_t q = base + tid;
-            PT p;
+            PT p = pn;
+            if (q + PUSH_THREADS < cnt) load_p(in, seg_index(bs, q + PUSH_THREADS), pn);
             bool alive = false;
-            int dtile = -1;
+            int dtile = -1, nbi = -1;  // destination tile, and its index among the 27 around t
             if (q < cnt) {
-                load_p(in, seg_index(bs, q), p);
+                int ci, cj, ck;  // cell after the push
                 if constexpr (PREC == PIC_MIXED) {
                     int li = min(max(p.ci - c0x, 0), TILE - 1);
                     int lj = min(max(p.cj - c0y, 0), TILE - 1);
                     int lk = min(max(p.ck - c0z, 0), TILE - 1);
-                    float3 E = gather_lds(reinterpret_cast<const float4 *>(sRaw), li, lj, lk, p.ox, p.oy, p.oz);
+                    float3 E = gather_lds(reinterpret_cast<const float *>(sRaw), li, lj, lk, p.ox, p.oy, p.oz);
                     alive = push_mixed(P, p, E);
+                    ci = p.ci;
+                    cj = p.cj;
+                    ck = p.ck;
                     if (alive) {
                         int wi = p.ci - c0x + HALO, wj = p.cj - c0y + HALO, wk = p.ck - c0z + HALO;
                         if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO && wk >= 0 &&
@@ -643,34 +664,38 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                         else
                             deposit_global64(g, O.den_extra, p.ci, p.cj, p.ck, (double)p.ox, (double)p.oy,
                                              (double)p.oz, (double)p.w);
-                        dtile = tile_id(g, p.ci, p.cj, p.ck);
                     }
                 } else {
-                    int i, j, k;
                     double di, dj, dk;
-                    cell_of(g, p, i, j, k, di, dj, dk);
-                    int li = min(max(i - c0x, 0), TILE - 1);
-                    int lj = min(max(j - c0y, 0), TILE - 1);
-                    int lk = min(max(k - c0z, 0), TILE - 1);
+                    cell_of(g, p, ci, cj, ck, di, dj, dk);
+                    int li = min(max(ci - c0x, 0), TILE - 1);
+                    int lj = min(max(cj - c0y, 0), TILE - 1);
+                    int lk = min(max(ck - c0z, 0), TILE - 1);
                     const double *pl = reinterpret_cast<const double *>(sRaw);
                     double3 E = gather64_weights(pl, pl + TNODES, pl + 2 * TNODES, li + TN * (lj + TN * lk), TN,
                                                  TN * TN, di, dj, dk);
                     alive = push_f64(P, p, E);
                     if (alive) {
-                        cell_of(g, p, i, j, k, di, dj, dk);
-                        int wi = i - c0x + HALO, wj = j - c0y + HALO, wk = k - c0z + HALO;
+                        cell_of(g, p, ci, cj, ck, di, dj, dk);
+                        int wi = ci - c0x + HALO, wj = cj - c0y + HALO, wk = ck - c0z + HALO;
                         if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO && wk >= 0 &&
                             wk < TILE + 2 * HALO)
                             deposit_lds<double>(sW, wi, wj, wk, di, dj, dk, p.w);
                         else
-                            deposit_global64(g, O.den_extra, i, j, k, di, dj, dk, p.w);
-                        dtile = tile_id(g, i, j, k);
+                            deposit_global64(g, O.den_extra, ci, cj, ck, di, dj, dk, p.w);
                     }
                 }
-                if (alive) tally(p);
+                if (alive) {
+                    tally(p);
+                    dtile = tile_id(g, ci, cj, ck);
+                    int dx = (ci >> TILE_SHIFT) - tx, dy = (cj >> TILE_SHIFT) - ty, dz = (ck >> TILE_SHIFT) - tz;
+                    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
+                        nbi = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                }
             }
             // stayers: one LDS atomic per wavefront reserves their slots in the own segment
             bool stay = alive && dtile == t;
+            long idx = -1;
             unsigned long long m = __ballot(stay);
             if (m) {
                 uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -679,24 +704,24 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                 uint32_t wbase = 0;
                 if ((int)__lane_id() == leader) wbase = atomicAdd(&sOwn, (uint32_t)__popcll(m));
                 wbase = __shfl(wbase, leader, 64);
-                if (stay) {
-                    uint32_t slot = wbase + before;
-                    if (slot < own_cap) {
-                        store_p(out, own0 + slot, p);
-                    } else {  // bins shrunk to nothing by a short buffer: spill to staging
-                        uint32_t st = atomicAdd(out.stage_count, 1u);
-                        if (st < out.cap_stage)
-                            store_p(out, out.cap_bins + st, p);
-                        else
-                            atomicOr(O.err, 2u);
-                    }
-                }
+                uint32_t slot = wbase + before;
+                if (stay && slot < own_cap) idx = own0 + slot;
             }
             // arrivals elsewhere: wavefront-aggregated reservation in the destination's
             // inflow segment
             int key = (alive && !stay) ? dtile : -1;
             uint32_t pos = wave_reserve_global(key, out.bin_in);
-            if (key >= 0) store_inflow(out, p, key, pos, O.err);
+            if (key >= 0) {
+                if (nbi >= 0) {
+                    uint32_t off = sNb[1][nbi], cap = sNb[2][nbi];
+                    idx = pos < cap - off ? (long)sNb[0][nbi] + off + pos : -1L;
+                } else {
+                    idx = inflow_slot(out, key, pos);
+                }
+            }
+            // no room (bins shrunk by a short buffer, inflow segment full): staging list
+            if (alive && idx < 0) idx = staging_slot(out, O.err);
+            if (idx >= 0) store_p(out, idx, p);
         }
         __syncthreads();
         // flush the deposit window of this tile
@@ -754,7 +779,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
     // per-block partials of getRealCount, getMomentum, getKE and the particle count
 #pragma unroll
     for (int q = 0; q < NDIAG_SLOTS; q++) {
-        double v = warp_sum(acc[q]);
+        double v = warp_sum((double)acc[q]);
         if ((tid & 63) == 0) sRed[q][tid >> 6] = v;
     }
     __syncthreads();

@@ -319,15 +319,37 @@ __global__ void __launch_bounds__(256)
 // with M g = (1.0/jd) g as Matrix::invDiagonal.  d ping-pongs between two buffers so that
 // pass A reads only old values.  The last deferred y update is applied by k_pcg_finish.
 
+// The PCG vector passes run on a fixed grid of at most PCG_GRID workgroups, each streaming
+// a contiguous chunk of whole 256-node strides and writing one partial, so the scalar steps
+// reduce at most PCG_GRID values.  Workgroups are dispatched round-robin over the 8 XCDs;
+// with a multiple of 8 workgroups, XCD x gets chunks [x G/8, (x+1) G/8), a slab of
+// consecutive k planes, so the +-k stencil neighbours of a chunk are read by workgroups
+// sharing its L2.
+constexpr unsigned PCG_GRID = 2048;
+
+struct NodeChunk {
+    long u0, u1;
+};
+
+__device__ inline NodeChunk pcg_chunk(long nn) {
+    unsigned G = gridDim.x, b = blockIdx.x;
+    unsigned c = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    long per = (nn + G - 1) / G;
+    per = (per + 255) / 256 * 256;
+    long u0 = (long)c * per;
+    long u1 = u0 + per < nn ? u0 + per : nn;
+    return NodeChunk{u0, u1};
+}
+
 // g = J y - F; d_0 = 0; partials of g.s  (s = M g)
 __global__ void __launch_bounds__(256)
     k_pcg_init(PcgParams Q, const uint8_t *oid, const double *jd, const double *y, const double *F, double *gv,
                double *d0, double *partials) {
     __shared__ double sh[4];
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
+    NodeChunk ch = pcg_chunk(g.nn);
     double gs = 0;
-    if (u < g.nn) {
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
         int i, j, k;
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
@@ -335,7 +357,7 @@ __global__ void __launch_bounds__(256)
         double s = (1.0 / jd[u]) * gg;
         gv[u] = gg;
         d0[u] = 0;
-        gs = gg * s;
+        gs += gg * s;
     }
     double r = block_sum(gs, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
@@ -350,11 +372,11 @@ __global__ void __launch_bounds__(256)
             double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
+    NodeChunk ch = pcg_chunk(g.nn);
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
-    if (u < g.nn) {
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
         int i, j, k;
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
@@ -388,7 +410,7 @@ __global__ void __launch_bounds__(256)
         }
         d_new[u] = dc;
         z[u] = zz;
-        dz = dc * zz;
+        dz += dc * zz;
     }
     double r = block_sum(dz, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
@@ -399,15 +421,15 @@ __global__ void __launch_bounds__(256)
             const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    NodeChunk ch = pcg_chunk(Q.g.nn);
     double alpha = sc->red[0], beta = sc->red[1];
     double gs = 0, g2 = 0;
-    if (u < Q.g.nn) {
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
         double gg = gv[u] + (alpha / beta) * z[u];
         gv[u] = gg;
         double s = (1.0 / jd[u]) * gg;
-        gs = gg * s;
-        g2 = gg * gg;
+        gs += gg * s;
+        g2 += gg * gg;
     }
     double r1 = block_sum(gs, sh);
     double r2 = block_sum(g2, sh);
@@ -623,14 +645,14 @@ __global__ void __launch_bounds__(256)
     k_pcgs_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
                 double *d0, double *partials) {
     __shared__ double sh[4];
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    NodeChunk ch = pcg_chunk(Q.g.nn);
     double gs = 0;
-    if (u < Q.g.nn) {
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
         uint8_t m = mask[u];
         double gg = 0;
         if (m & 64) {
             gg = jr_apply(Q, u, m, jr[u], y) - F[u];
-            gs = gg * ((1.0 / jr[u]) * gg);
+            gs += gg * ((1.0 / jr[u]) * gg);
         }
         gv[u] = gg;
         d0[u] = 0;
@@ -644,11 +666,11 @@ __global__ void __launch_bounds__(256)
              double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
+    NodeChunk ch = pcg_chunk(g.nn);
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
-    if (u < g.nn) {
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
         uint8_t m = mask[u];
         if (m & 64) {
             y[u] = y[u] + lam * d_old[u];
@@ -667,7 +689,7 @@ __global__ void __launch_bounds__(256)
                 }
             d_new[u] = dc;
             z[u] = zz;
-            dz = dc * zz;
+            dz += dc * zz;
         } else {
             d_new[u] = 0;
             z[u] = 0;
@@ -841,23 +863,24 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
+    unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // vector-pass workgroups = partials
     int rc = grow_partials(w, (int)nb);
     if (rc) return rc;
     double nu = (double)(int)nn;
     double *dbuf[2] = {s->d, s->z2};
     k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
-    k_pcg_init<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, s->g, dbuf[0], w->partials);
-    k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 0, nu, Q.tol, 0, w->scal);
+    k_pcg_init<<<G, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->y, s->F, s->g, dbuf[0], w->partials);
+    k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 0, nu, Q.tol, 0, w->scal);
     const int CHUNK = 32;
     bool conv = false;
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_pcg_a<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+            k_pcg_a<<<G, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
                                               s->z, w->partials, w->scal);
-            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 1, nu, Q.tol, it, w->scal);
-            k_pcg_b<<<nb, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)nb, w->scal);
-            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 2, nu, Q.tol, it, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 1, nu, Q.tol, it, w->scal);
+            k_pcg_b<<<G, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)G, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
         }
         PIC_LAUNCH_CHECK();
         rc = read_scalars(w);
@@ -963,21 +986,22 @@ static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
+    unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // vector-pass workgroups = partials
     double nu = (double)(int)nn;
     double *dbuf[2] = {s->d, s->z2};
     k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
-    k_pcgs_init<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0], w->partials);
-    k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 0, nu, Q.tol, 0, w->scal);
+    k_pcgs_init<<<G, 256, 0, w->stream>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0], w->partials);
+    k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 0, nu, Q.tol, 0, w->scal);
     const int CHUNK = 32;
     bool conv = false;
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_pcgs_a<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+            k_pcgs_a<<<G, 256, 0, w->stream>>>(Q, s->mask, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
                                                s->z, w->partials, w->scal);
-            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 1, nu, Q.tol, it, w->scal);
-            k_pcg_b<<<nb, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)nb, w->scal);
-            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, 2, nu, Q.tol, it, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 1, nu, Q.tol, it, w->scal);
+            k_pcg_b<<<G, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)G, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
         }
         PIC_LAUNCH_CHECK();
         int rc = read_scalars(w);
