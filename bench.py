@@ -5,7 +5,8 @@ Workload (default): flow past a sphere on a 257x257x513 grid over [-0.1,0.1]^2 x
 sphere r=0.05 m at (0,0,0.15) at -100 V, inlet k=0 at 0 V; 1e9 O+ macroparticles loaded
 uniformly with a 7 km/s drift along z plus a 1000 K isotropic thermal spread (Philox,
 seed 1), a warm beam injected at z=0 every step (n0=1e10 m^-3); Boltzmann electrons
-(n0=1e10, Te=1.5 eV, phi0=0) solved by Newton + Jacobi-PCG (tol 1e-4, NR tol 1e-3), dt=1e-7 s.
+(n0=1e10, Te=1.5 eV, phi0=0) solved by Newton + CG (tol 1e-4, NR tol 1e-3) on the
+Neumann-reduced system with a multigrid V-cycle as the preconditioner, dt=1e-7 s.
 
 One step = inject -> advance (fused push + CIC gather + CIC deposit + rebin) ->
 computeNumberDensity -> computeChargeDensity -> solve -> computeEF, the call sequence of
@@ -117,16 +118,16 @@ def initial_potential(P, w, solver_tol=1e-2):
     """Initial phi for a domain preloaded with a uniform plasma.  Undamped Newton from
     phi = 0 around a -100 V sphere diverges (the exp(phi/Te) term overshoots); the
     reference never meets that state because its domain fills from vacuum.  Continuation
-    in the sphere potential (-25, -50, -75, -100 V) with a loose-tolerance NR-PCG reaches
+    in the sphere potential (-25, -50, -75, -100 V) with a loose-tolerance Newton solve reaches
     the sheath solution; the timed steps then use the reference solver settings."""
-    s = P.PotentialSolver(w, P.PCG_SYM, 5000, solver_tol)
+    s = P.PotentialSolver(w, P.MG, 5000, solver_tol)
     s.set_reference_values(0.0, Case.Te, Case.n0)
     for k in range(1, 5):
         phi_s = Case.phi_s * k / 4
         w.add_sphere(Case.sphere_c, Case.sphere_r, phi_s)
         t0 = time.time()
         ok, res, its = s.solve()
-        log(f"[bench]   continuation phi_s={phi_s:.0f} V: converged {ok}, {its} PCG its, "
+        log(f"[bench]   continuation phi_s={phi_s:.0f} V: converged {ok}, {its} CG its, "
             f"{time.time() - t0:.1f}s")
     s.close()
 
@@ -153,10 +154,12 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     else:
         w.set_field(P.PHI, phi)
     phi_init = w.field(P.PHI)
-    # Newton + Jacobi-PCG with the reference's tolerances, on the Neumann-reduced system:
-    # the reference's one-sided Neumann rows leave a non-symmetric Jacobian on which CG
-    # stalls at this size (PIC_SOLVER_PCG_SYM reaches the same discrete solution)
-    solver = P.PotentialSolver(w, P.PCG_SYM, 10000, 1e-4)
+    # Newton + CG with the reference's tolerances, on the Neumann-reduced system (the
+    # reference's one-sided Neumann rows leave a non-symmetric Jacobian on which CG stalls
+    # at this size), preconditioned by a multigrid V-cycle (PIC_SOLVER_MG; the Jacobi
+    # preconditioner of the reference, PIC_SOLVER_PCG_SYM, reaches the same solution in
+    # ~100x more CG iterations)
+    solver = P.PotentialSolver(w, P.MG, 10000, 1e-4)
     solver.set_reference_values(0.0, Case.Te, Case.n0)
     solver.compute_ef()
 
@@ -176,7 +179,7 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
         f"particles in {t_load:.1f}s; warmup {warmup}")
     for i in range(warmup):
         ok, its = step()
-        log(f"[bench]   warmup {i}: pcg its {its} converged {ok} np {ions.np():.4g}")
+        log(f"[bench]   warmup {i}: cg its {its} converged {ok} np {ions.np():.4g}")
     os.environ["PIC_VERBOSE"] = "0"
     w.set_timing(True)
     w.timers(reset=True)
@@ -189,7 +192,7 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
         ok, its = step()
         np_sum += n_before
         pcg.append(its)
-        log(f"[bench]   step {i}: pcg its {its} converged {ok}")
+        log(f"[bench]   step {i}: cg its {its} converged {ok}")
     w.sync()
     wall = time.perf_counter() - t0
     tm = w.timers(reset=True)
@@ -324,14 +327,14 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"sphere {grid[0]}x{grid[1]}x{grid[2]}, {args.particles:.3g} O+ "
-                        f"mixed-precision particles, Boltzmann-electron Newton-PCG",
+                        f"mixed-precision particles, Boltzmann-electron Newton-MGPCG",
             "particles": args.particles, "grid": list(grid), "dt": Case.dt,
-            "solver": "Newton + Jacobi-PCG on the Neumann-reduced system (PIC_SOLVER_PCG_SYM), "
-                      "FP64, tol 1e-4, NR tol 1e-3",
+            "solver": "Newton + multigrid-preconditioned CG on the Neumann-reduced system "
+                      "(PIC_SOLVER_MG), FP64, tol 1e-4, NR tol 1e-3",
             "particles_final": mixed["np_final"],
         },
         "poisson_ms_per_step": mixed["solve_ms"] / steps,
-        "pcg_iters_per_step": float(np.mean(mixed["pcg_iters"])),
+        "cg_iters_per_step": float(np.mean(mixed["pcg_iters"])),
         "phase_ms_per_step": {k: mixed[k + "_ms"] / steps for k in
                               ("push", "deposit", "inject", "solve", "ef", "rho")},
         "roofline": roofline,

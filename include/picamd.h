@@ -59,6 +59,11 @@ extern "C" {
 #define PIC_SOLVER_QN 2
 #define PIC_SOLVER_SOR_LINEAR 3
 #define PIC_SOLVER_PCG_SYM 4
+/* MG: the Newton iteration and stopping tests of PCG_SYM with CG preconditioned by one
+ * geometric-multigrid V-cycle instead of the diagonal (Matrix::invDiagonal,
+ * ch3/ver2/PotentialSolver.cpp:52-57): the same discrete solution in a number of CG
+ * iterations that barely grows with the grid. */
+#define PIC_SOLVER_MG 5
 
 /* world node fields for pic_world_get_field / pic_world_set_field */
 #define PIC_FIELD_PHI 0       /* World::phi */

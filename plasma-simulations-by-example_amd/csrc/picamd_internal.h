@@ -184,6 +184,10 @@ struct pic_species {
     long stage_added = 0;         // particles appended to staging since the last push or sort
 };
 
+namespace pic {
+struct MgHierarchy;
+}
+
 struct pic_solver {
     pic_world *w = nullptr;
     int type = PIC_SOLVER_GS;
@@ -194,6 +198,9 @@ struct pic_solver {
     // (P holds the Jacobian diagonal jd = A(u,u) - P(u); d and z2 are the ping-pong pair of d)
     double *y = nullptr, *g = nullptr, *d = nullptr, *z = nullptr, *F = nullptr, *P = nullptr, *z2 = nullptr;
     uint8_t *mask = nullptr;  // PCG_SYM: regular-neighbour bits per node
+    // PIC_SOLVER_MG: preconditioned residual s, V-cycle scratch, Boltzmann derivative, levels
+    double *sv = nullptr, *rv = nullptr, *boltz = nullptr;
+    pic::MgHierarchy *mg = nullptr;
 };
 
 // ---- error handling ------------------------------------------------------------------

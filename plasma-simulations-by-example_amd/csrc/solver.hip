@@ -11,6 +11,7 @@
 #include <cstdlib>
 
 #include "picamd_internal.h"
+#include "stencil.h"
 
 #pragma clang fp contract(off)
 
@@ -28,12 +29,6 @@ struct SorParams {
     int max_it;
 };
 
-__device__ inline void node_ijk(const Grid &g, long u, int &i, int &j, int &k) {
-    i = (int)(u % g.ni);
-    long r = u / g.ni;
-    j = (int)(r % g.nj);
-    k = (int)(r / g.nj);
-}
 
 // ---- one node update -------------------------------------------------------------------
 // linear SOR, interior nodes only (ch2/PotentialSolver.cpp:32-42)
@@ -208,11 +203,6 @@ __global__ void __launch_bounds__(256)
 }
 
 // one workgroup: sums nb partials per slot in a fixed order
-__device__ inline double fixed_sum(const double *p, int nb, double *sh) {
-    double s = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += p[b];
-    return block_sum(s, sh);
-}
 
 __global__ void __launch_bounds__(1024)
     k_sor_check(const double *partials, int nb, double denom_nodes, double tol, int it, DevScalars *sc) {
@@ -232,20 +222,9 @@ __global__ void __launch_bounds__(1024)
 // ---- Newton-PCG, matrix-free -----------------------------------------------------------
 // Row of A (buildMatrix, ch3/ver2/PotentialSolver.cpp:146-201) applied to v in the slot
 // order of the reference matrix; diag is the row's diagonal (A's, or J = A - diag(P)).
-enum { NT_REG = 0, NT_NEU = 1, NT_DIR = 2 };
 
-struct PcgParams {
-    Grid g;
-    double idx, idy, idz, idx2, idy2, idz2;
-    double phi0, Te0, n0;
-    double tol;
-};
 
-__device__ inline int node_type(const Grid &g, const uint8_t *oid, long u, int i, int j, int k) {
-    if (oid[u] > 0) return NT_DIR;
-    if (i == 0 || i == g.ni - 1 || j == 0 || j == g.nj - 1 || k == 0 || k == g.nk - 1) return NT_NEU;
-    return NT_REG;
-}
+
 
 // A's diagonal
 __device__ inline double a_diag(const PcgParams &Q, int type, int i, int j) {
@@ -319,13 +298,27 @@ __global__ void __launch_bounds__(256)
 // with M g = (1.0/jd) g as Matrix::invDiagonal.  d ping-pongs between two buffers so that
 // pass A reads only old values.  The last deferred y update is applied by k_pcg_finish.
 
-// The PCG vector passes run on a fixed grid of at most PCG_GRID workgroups, each streaming
-// a contiguous chunk of whole 256-node strides and writing one partial, so the scalar steps
-// reduce at most PCG_GRID values.  Workgroups are dispatched round-robin over the 8 XCDs;
-// with a multiple of 8 workgroups, XCD x gets chunks [x G/8, (x+1) G/8), a slab of
-// consecutive k planes, so the +-k stencil neighbours of a chunk are read by workgroups
-// sharing its L2.
+// The streaming PCG passes (init, pass B) run on a fixed grid of at most PCG_GRID
+// workgroups, each walking a contiguous chunk of whole 256-node strides and writing one
+// partial, so the scalar step reduces at most PCG_GRID values.  Workgroups are dispatched
+// round-robin over the 8 XCDs; with a multiple of 8 workgroups, XCD x gets chunks
+// [x G/8, (x+1) G/8), a slab of consecutive k planes.  The stencil pass A keeps one node per
+// thread (its gathers need the latency hiding of many resident wavefronts); its per-block
+// partials are first folded to FOLD_GRID values by k_fold.
 constexpr unsigned PCG_GRID = 2048;
+constexpr unsigned FOLD_GRID = 128;
+
+// out[b] = sum of the b-th of FOLD_GRID contiguous slices of in[0, n), fixed order
+__global__ void __launch_bounds__(256) k_fold(const double *in, long n, double *out, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long per = (n + gridDim.x - 1) / gridDim.x;
+    long a = blockIdx.x * per, b = a + per < n ? a + per : n;
+    double s = 0;
+    for (long i = a + threadIdx.x; i < b; i += 256) s += in[i];
+    double r = block_sum(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
 
 struct NodeChunk {
     long u0, u1;
@@ -373,10 +366,10 @@ __global__ void __launch_bounds__(256)
     __shared__ double sh[4];
     if (sc->done) return;
     const Grid &g = Q.g;
-    NodeChunk ch = pcg_chunk(g.nn);
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
-    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
+    if (u < g.nn) {
         int i, j, k;
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
@@ -410,7 +403,7 @@ __global__ void __launch_bounds__(256)
         }
         d_new[u] = dc;
         z[u] = zz;
-        dz += dc * zz;
+        dz = dc * zz;
     }
     double r = block_sum(dz, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
@@ -556,32 +549,6 @@ __global__ void k_fill(long n, double *p, double v) {
 // rows enforce at convergence).  For a regular node every Neumann neighbour reaches the
 // node itself, so eliminating them adds their coefficient to the diagonal: the reduced
 // Jacobian is symmetric and negative definite.  Vectors are zero off the regular nodes.
-__device__ inline long chain_target(const Grid &g, const uint8_t *oid, long u, int i, int j, int k) {
-    long sj = g.ni, sk = (long)g.ni * g.nj;
-    for (int hop = 0; hop < 3; hop++) {
-        if (oid[u] > 0) return u;
-        if (i == 0) { i = 1; u += 1; }
-        else if (i == g.ni - 1) { i = g.ni - 2; u -= 1; }
-        else if (j == 0) { j = 1; u += sj; }
-        else if (j == g.nj - 1) { j = g.nj - 2; u -= sj; }
-        else if (k == 0) { k = 1; u += sk; }
-        else if (k == g.nk - 1) { k = g.nk - 2; u -= sk; }
-        else return u;
-    }
-    return u;
-}
-
-// direction d: 0 -z, 1 -y, 2 -x, 3 +x, 4 +y, 5 +z (the reference's slot order around the centre)
-__device__ inline long nb_offset(const Grid &g, int d) {
-    long sj = g.ni, sk = (long)g.ni * g.nj;
-    const long o[6] = {-sk, -sj, -1, 1, sj, sk};
-    return o[d];
-}
-
-__device__ inline double nb_coef(const PcgParams &Q, int d) {
-    const double c[6] = {Q.idz2, Q.idy2, Q.idx2, Q.idx2, Q.idy2, Q.idz2};
-    return c[d];
-}
 
 // x on face nodes := x at their chain target
 __global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid, double *x) {
@@ -598,7 +565,7 @@ __global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid
 // bits 0-5 regular neighbours in direction d, bit 6 the node itself is regular.
 __global__ void __launch_bounds__(256)
     k_nrs_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *jr,
-            uint8_t *mask) {
+            uint8_t *mask, double *boltz) {
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
@@ -609,6 +576,7 @@ __global__ void __launch_bounds__(256)
         F[u] = 0;
         jr[u] = 1;
         mask[u] = 0;
+        if (boltz) boltz[u] = 0;
         return;
     }
     double ad = a_diag(Q, t, i, j);
@@ -628,18 +596,9 @@ __global__ void __launch_bounds__(256)
     F[u] = f;
     jr[u] = diag;
     mask[u] = m;
+    if (boltz) boltz[u] = p;
 }
 
-__device__ inline double jr_apply(const PcgParams &Q, long u, uint8_t m, double jru, const double *v) {
-    const Grid &g = Q.g;
-    double s = 0;
-    for (int d = 0; d < 3; d++)
-        if (m & (1u << d)) s += nb_coef(Q, d) * v[u + nb_offset(g, d)];
-    s += jru * v[u];
-    for (int d = 3; d < 6; d++)
-        if (m & (1u << d)) s += nb_coef(Q, d) * v[u + nb_offset(g, d)];
-    return s;
-}
 
 __global__ void __launch_bounds__(256)
     k_pcgs_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
@@ -667,10 +626,10 @@ __global__ void __launch_bounds__(256)
     __shared__ double sh[4];
     if (sc->done) return;
     const Grid &g = Q.g;
-    NodeChunk ch = pcg_chunk(g.nn);
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
-    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
+    if (u < g.nn) {
         uint8_t m = mask[u];
         if (m & 64) {
             y[u] = y[u] + lam * d_old[u];
@@ -689,7 +648,7 @@ __global__ void __launch_bounds__(256)
                 }
             d_new[u] = dc;
             z[u] = zz;
-            dz += dc * zz;
+            dz = dc * zz;
         } else {
             d_new[u] = 0;
             z[u] = 0;
@@ -697,6 +656,115 @@ __global__ void __launch_bounds__(256)
     }
     double r = block_sum(dz, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// ---- CG preconditioned by one multigrid V-cycle (PIC_SOLVER_MG) -------------------------
+// The iteration of solvePCGLinear (:299-331) with s = V(g) in place of s = M g:
+//   init:    g = J y - F;  s = V(g);  a = g.s;  d_0 = 0
+//   pass A:  d = mu d_old - s (at the 7 stencil points), y += lam d_old, z = J d, partials d.z
+//   pass B:  g += (a/b) z, partials g.g (convergence: RMS(g) < tol)
+//   s = V(g), partials g.s:  mu = a'/a
+// Scalars: red[0] a, red[1] b, red[4] RMS(g), red[6] mu, red[7] lam = a/b.
+__global__ void __launch_bounds__(256)
+    k_mgpcg_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
+                 double *d0) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u >= Q.g.nn) return;
+    uint8_t m = mask[u];
+    gv[u] = (m & 64) ? jr_apply(Q, u, m, jr[u], y) - F[u] : 0.0;
+    d0[u] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+    k_mgpcg_dot(long nn, const double *a, const double *b, double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    NodeChunk ch = pcg_chunk(nn);
+    double s = 0;
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) s += a[u] * b[u];
+    double r = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(256)
+    k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const double *sv, const double *d_old,
+              double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    const Grid &g = Q.g;
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double mu = sc->red[6], lam = sc->red[7];
+    double dz = 0;
+    if (u < g.nn) {
+        uint8_t m = mask[u];
+        if (m & 64) {
+            y[u] = y[u] + lam * d_old[u];
+            double dc = mu * d_old[u] - sv[u];
+            double zz = 0;
+            for (int d = 0; d < 3; d++)
+                if (m & (1u << d)) {
+                    long v = u + nb_offset(g, d);
+                    zz += nb_coef(Q, d) * (mu * d_old[v] - sv[v]);
+                }
+            zz += jr[u] * dc;
+            for (int d = 3; d < 6; d++)
+                if (m & (1u << d)) {
+                    long v = u + nb_offset(g, d);
+                    zz += nb_coef(Q, d) * (mu * d_old[v] - sv[v]);
+                }
+            d_new[u] = dc;
+            z[u] = zz;
+            dz = dc * zz;
+        } else {
+            d_new[u] = 0;
+            z[u] = 0;
+        }
+    }
+    double r = block_sum(dz, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(256)
+    k_mgpcg_b(long nn, double *gv, const double *z, double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    NodeChunk ch = pcg_chunk(nn);
+    double lam = sc->red[7];
+    double g2 = 0;
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
+        double gg = gv[u] + lam * z[u];
+        gv[u] = gg;
+        g2 += gg * gg;
+    }
+    double r = block_sum(g2, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// mode 0: a = g.s after init;  mode 1: b = d.z, lam = a/b;  mode 2: RMS(g), convergence;
+// mode 3: a' = g.s, mu = a'/a
+__global__ void __launch_bounds__(1024)
+    k_mgpcg_scalar(const double *partials, int nb, int mode, double nu, double tol, int it, DevScalars *sc) {
+    __shared__ double sh[16];
+    if (mode != 0 && sc->done) return;
+    double s0 = fixed_sum(partials, nb, sh);
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        sc->red[0] = s0;
+        sc->red[6] = 0.0;
+        sc->red[7] = 0.0;
+        sc->iters = 0;
+    } else if (mode == 1) {
+        sc->red[1] = s0;
+        sc->red[7] = sc->red[0] / s0;
+    } else if (mode == 2) {
+        double l2 = sqrt(s0 / nu);
+        sc->red[4] = l2;
+        sc->iters = it + 1;
+        if (l2 < tol) sc->done = 1;
+    } else {
+        sc->red[6] = s0 / sc->red[0];
+        sc->red[0] = s0;
+    }
 }
 
 // red-black Gauss-Seidel (w = 1) on the reduced system: the solveGSLinear fallback
@@ -756,7 +824,7 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
-static unsigned nbk(long n) { return (unsigned)((n + 255) / 256); }
+
 
 static SorParams sor_params(const pic_solver *s) {
     const Grid &g = s->w->g;
@@ -863,9 +931,10 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
-    unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // vector-pass workgroups = partials
+    unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // streaming-pass workgroups = partials
     int rc = grow_partials(w, (int)nb);
     if (rc) return rc;
+    double *folded = w->partials + 4 * (size_t)w->partials_cap;
     double nu = (double)(int)nn;
     double *dbuf[2] = {s->d, s->z2};
     k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
@@ -876,9 +945,10 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_pcg_a<<<G, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+            k_pcg_a<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
                                               s->z, w->partials, w->scal);
-            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 1, nu, Q.tol, it, w->scal);
+            k_fold<<<FOLD_GRID, 256, 0, w->stream>>>(w->partials, nb, folded, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_pcg_b<<<G, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)G, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
         }
@@ -986,7 +1056,10 @@ static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
-    unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // vector-pass workgroups = partials
+    unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // streaming-pass workgroups = partials
+    int rc = grow_partials(w, (int)nb);
+    if (rc) return rc;
+    double *folded = w->partials + 4 * (size_t)w->partials_cap;
     double nu = (double)(int)nn;
     double *dbuf[2] = {s->d, s->z2};
     k_reset_flags<<<1, 1, 0, w->stream>>>(w->scal);
@@ -997,9 +1070,10 @@ static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_pcgs_a<<<G, 256, 0, w->stream>>>(Q, s->mask, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+            k_pcgs_a<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
                                                s->z, w->partials, w->scal);
-            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 1, nu, Q.tol, it, w->scal);
+            k_fold<<<FOLD_GRID, 256, 0, w->stream>>>(w->partials, nb, folded, w->scal);
+            k_pcg_scalar<<<1, 1024, 0, w->stream>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_pcg_b<<<G, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)G, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
         }
@@ -1044,11 +1118,83 @@ static int gsls_linear(pic_solver *s, const PcgParams &Q) {
 }
 
 // Newton iteration of solveNRPCG (:225-296) on the Neumann-reduced system
+int mg_build(pic_solver *s, MgHierarchy **out);
+int mg_setup(MgHierarchy *H, const PcgParams &Q0, hipStream_t st);
+int mg_vcycle(MgHierarchy *H, hipStream_t st, const DevScalars *sc);
+void mg_destroy(MgHierarchy *H);
+
+// solvePCGLinear on the reduced system with one V-cycle as the preconditioner
+static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *its) {
+    pic_world *w = s->w;
+    long nn = Q.g.nn;
+    unsigned nb = nbk(nn);
+    unsigned G = nb < PCG_GRID ? nb : PCG_GRID;
+    int rc = grow_partials(w, (int)nb);
+    if (rc) return rc;
+    double *folded = w->partials + 4 * (size_t)w->partials_cap;
+    double nu = (double)(int)nn;
+    double *dbuf[2] = {s->d, s->z2};
+    hipStream_t st = w->stream;
+    k_reset_flags<<<1, 1, 0, st>>>(w->scal);
+    k_mgpcg_init<<<nb, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0]);
+    rc = mg_vcycle(s->mg, st, w->scal);
+    if (rc) return rc;
+    k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, s->sv, w->partials, w->scal);
+    k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 0, nu, Q.tol, 0, w->scal);
+    // a few iterations are queued between host checks of the convergence flag; kernels of
+    // iterations past convergence return at once
+    const int CHUNK = 2;
+    bool conv = false;
+    for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
+        int it1 = std::min(s->max_it, it0 + CHUNK);
+        for (int it = it0; it < it1; it++) {
+            k_mgpcg_a<<<nb, 256, 0, st>>>(Q, s->mask, s->P, s->sv, dbuf[it & 1], dbuf[(it + 1) & 1], s->y, s->z,
+                                         w->partials, w->scal);
+            k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, nb, folded, w->scal);
+            k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
+            k_mgpcg_b<<<G, 256, 0, st>>>(nn, s->g, s->z, w->partials, w->scal);
+            k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
+            rc = mg_vcycle(s->mg, st, w->scal);
+            if (rc) return rc;
+            k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, s->sv, w->partials, w->scal);
+            k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 3, nu, Q.tol, it, w->scal);
+        }
+        PIC_LAUNCH_CHECK();
+        rc = read_scalars(w);
+        if (rc) return rc;
+        if (w->scal_host->done) {
+            conv = true;
+            break;
+        }
+    }
+    int done_its = w->scal_host->iters;
+    k_pcg_finish<<<nb, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    PIC_LAUNCH_CHECK();
+    *its += done_its;
+    *converged = conv;
+    return PIC_OK;
+}
+
+static int ensure_mg(pic_solver *s) {
+    if (!s->sv) {
+        size_t n = (size_t)s->w->g.nn;
+        PIC_HIP(hipMalloc(&s->sv, n * sizeof(double)));
+        PIC_HIP(hipMalloc(&s->rv, n * sizeof(double)));
+        PIC_HIP(hipMalloc(&s->boltz, n * sizeof(double)));
+    }
+    return mg_build(s, &s->mg);
+}
+
 static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     pic_world *w = s->w;
     int rc = ensure_pcg_vectors(s);
     if (rc) return rc;
     if (!s->mask) PIC_HIP(hipMalloc(&s->mask, w->g.nn));
+    const bool use_mg = s->type == PIC_SOLVER_MG;
+    if (use_mg) {
+        rc = ensure_mg(s);
+        if (rc) return rc;
+    }
     PcgParams Q = pcg_params(s);
     long nn = w->g.nn;
     unsigned nb = nbk(nn);
@@ -1065,10 +1211,16 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     double norm = 0;
     int pcg_its = 0;
     for (int it = 0; it < NR_MAX_IT; it++) {
-        k_nrs_F<<<nb, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask);
+        k_nrs_F<<<nb, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz);
         PIC_LAUNCH_CHECK();
         bool pc = false;
-        rc = pcgs_linear(s, Q, &pc, &pcg_its);
+        if (use_mg) {
+            rc = mg_setup(s->mg, Q, w->stream);
+            if (rc) return rc;
+            rc = mgpcg_linear(s, Q, &pc, &pcg_its);
+        } else {
+            rc = pcgs_linear(s, Q, &pc, &pcg_its);
+        }
         if (rc) return rc;
         if (!pc) {
             rc = gsls_linear(s, Q);
@@ -1081,7 +1233,8 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
         if (rc) return rc;
         norm = sqrt(w->scal_host->red[5] / (double)(int)nn);
         if (verbose)
-            fprintf(stderr, "[picamd] NR(sym) it %d: PCG %s after %d its total, |y|rms %.3e, %.1f ms\n", it,
+            fprintf(stderr, "[picamd] NR(%s) it %d: PCG %s after %d its total, |y|rms %.3e, %.1f ms\n",
+                    use_mg ? "mg" : "sym", it,
                     pc ? "converged" : "fell back to GS", pcg_its, norm,
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         if (norm < NR_TOL) {
@@ -1103,7 +1256,7 @@ extern "C" {
 
 int pic_solver_create(pic_world *w, int type, int max_it, double tol, pic_solver **out) {
     PIC_CHECK(w && out, PIC_ERR_ARG, "null argument");
-    PIC_CHECK(type >= PIC_SOLVER_GS && type <= PIC_SOLVER_PCG_SYM, PIC_ERR_ARG, "bad solver type");
+    PIC_CHECK(type >= PIC_SOLVER_GS && type <= PIC_SOLVER_MG, PIC_ERR_ARG, "bad solver type");
     PIC_CHECK(max_it >= 0, PIC_ERR_ARG, "negative max_it");
     PIC_CHECK(w->extents_set, PIC_ERR_STATE, "setExtents must precede the solver");
     pic_solver *s = new pic_solver();
@@ -1126,6 +1279,10 @@ int pic_solver_destroy(pic_solver *s) {
     hipFree(s->P);
     hipFree(s->z2);
     hipFree(s->mask);
+    mg_destroy(s->mg);
+    hipFree(s->sv);
+    hipFree(s->rv);
+    hipFree(s->boltz);
     delete s;
     return PIC_OK;
 }
@@ -1149,7 +1306,8 @@ int pic_solver_solve(pic_solver *s, double *residual, int *iters) {
     case PIC_SOLVER_SOR_LINEAR: rc = solve_sor<false>(s, residual, iters); break;
     case PIC_SOLVER_GS: rc = solve_sor<true>(s, residual, iters); break;
     case PIC_SOLVER_PCG: rc = solve_nrpcg(s, residual, iters); break;
-    case PIC_SOLVER_PCG_SYM: rc = solve_nrpcg_sym(s, residual, iters); break;
+    case PIC_SOLVER_PCG_SYM:
+    case PIC_SOLVER_MG: rc = solve_nrpcg_sym(s, residual, iters); break;
     default:
         rc = solve_qn(w, s->phi0, s->Te0, s->n0);
         if (residual) *residual = 0;

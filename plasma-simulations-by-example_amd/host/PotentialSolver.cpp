@@ -9,7 +9,11 @@ using namespace std;
 
 PotentialSolver::PotentialSolver(World &world, SolverType type, int max_it, double tol)
     : world(world),
-      kind(type == GS ? PIC_SOLVER_GS : type == PCG ? PIC_SOLVER_PCG : type == PCGS ? PIC_SOLVER_PCG_SYM : PIC_SOLVER_QN) {
+      kind(type == GS     ? PIC_SOLVER_GS
+           : type == PCG  ? PIC_SOLVER_PCG
+           : type == PCGS ? PIC_SOLVER_PCG_SYM
+           : type == MG   ? PIC_SOLVER_MG
+                          : PIC_SOLVER_QN) {
     pic_check(pic_solver_create(world.dev(), kind, max_it, tol, &handle), "pic_solver_create");
     buildMatrix();
 }
@@ -32,7 +36,7 @@ void PotentialSolver::buildMatrix() { pic_check(pic_solver_solve_qn(handle), "so
 bool PotentialSolver::solve() {
     int rc = pic_solver_solve(handle, &last_res, &last_its);
     if (rc == PIC_NOT_CONVERGED) {
-        if (kind == PIC_SOLVER_PCG || kind == PIC_SOLVER_PCG_SYM)
+        if (kind == PIC_SOLVER_PCG || kind == PIC_SOLVER_PCG_SYM || kind == PIC_SOLVER_MG)
             cout << "NR+PCG failed to converge, norm = " << last_res << endl;
         else
             cerr << "GS failed to converge, L2=" << last_res << endl;

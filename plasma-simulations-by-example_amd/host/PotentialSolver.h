@@ -8,8 +8,9 @@
 
 #include "World.h"
 
-// PCGS: Newton + PCG on the Neumann-reduced (symmetric) Jacobian, PIC_SOLVER_PCG_SYM
-enum SolverType { GS, PCG, QN, PCGS };
+// PCGS: Newton + PCG on the Neumann-reduced (symmetric) Jacobian, PIC_SOLVER_PCG_SYM;
+// MG: the same Newton iteration with multigrid-preconditioned CG, PIC_SOLVER_MG
+enum SolverType { GS, PCG, QN, PCGS, MG };
 
 class PotentialSolver {
 public:

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libpicamd.so")
 PIC_OK, PIC_NOT_CONVERGED = 0, 1
 MIXED, FP64 = 0, 1
 REFLECT, ABSORB = 0, 1
-GS, PCG, QN, SOR_LINEAR, PCG_SYM = 0, 1, 2, 3, 4
+GS, PCG, QN, SOR_LINEAR, PCG_SYM, MG = 0, 1, 2, 3, 4, 5
 PHI, RHO, NODE_VOL, EF, OBJECT_ID = 0, 1, 2, 3, 4
 T_PUSH, T_DEPOSIT, T_SORT, T_SOLVE, T_EF, T_RHO, T_INJECT = range(7)
 T_NAMES = ["push", "deposit", "sort", "solve", "ef", "rho", "inject"]

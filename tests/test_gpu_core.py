@@ -276,3 +276,54 @@ def test_nrpcg_sym_same_solution_as_reference_nrpcg():
     ok_r, st = O.solve_nrpcg(g, ref, rho, oid, 0, 1.5, 1e10, 10000, 1e-8)
     assert ok_r
     np.testing.assert_allclose(w.field(P.PHI), ref, rtol=0, atol=2e-3)
+
+
+def test_mg_same_solution_as_reference_nrpcg():
+    """PIC_SOLVER_MG (multigrid-preconditioned CG inside the same Newton iteration) reaches
+    the reference NR-PCG's solution, and the PCG_SYM solution of the same reduced system."""
+    w, g, oid, phi, rho = _sphere_world()
+    O.solve_qn(g, phi, g.zeros(), oid, 0, 1.5, 1e12)
+    w.set_field(P.PHI, phi)
+    w.set_field(P.RHO, rho)
+    s = P.PotentialSolver(w, P.MG, 10000, 1e-8)
+    s.set_reference_values(0, 1.5, 1e10)
+    ok, res, its = s.solve()
+    assert ok and res < 1e-3
+    mg = w.field(P.PHI)
+    ref = phi.copy()
+    ok_r, st = O.solve_nrpcg(g, ref, rho, oid, 0, 1.5, 1e10, 10000, 1e-8)
+    assert ok_r
+    np.testing.assert_allclose(mg, ref, rtol=0, atol=2e-3)
+    w.set_field(P.PHI, phi)
+    s2 = P.PotentialSolver(w, P.PCG_SYM, 10000, 1e-8)
+    s2.set_reference_values(0, 1.5, 1e10)
+    ok2, _, its2 = s2.solve()
+    assert ok2
+    assert rel_l2(mg, w.field(P.PHI)) < 1e-8
+    assert its < its2 / 3, (its, its2)
+
+
+@pytest.mark.parametrize("n", [(21, 21, 21), (33, 33, 65), (65, 65, 129)])
+def test_mg_iterations_grid_independent(n):
+    """CG iterations with the V-cycle stay nearly constant as the grid is refined (Jacobi
+    PCG's grow with the grid width), on grids that coarsen fully (33x33x65 -> 3x3x5) and on
+    one whose coarsening stops early (21^3 -> 11^3 -> 6^3); same solution as PCG_SYM."""
+    x0, xm = (-0.1, -0.1, 0.0), (0.1, 0.1, 0.4)
+    rng = np.random.default_rng(5)
+    nn = n[0] * n[1] * n[2]
+    rho = rng.uniform(0.5, 1.5, nn) * 1e10 * O.QE
+    out = {}
+    for typ in (P.MG, P.PCG_SYM):
+        w, g = make_world(n, x0, xm)
+        w.add_sphere((0, 0, 0.15), 0.05, -100.0)
+        w.add_inlet()
+        w.set_field(P.RHO, rho)
+        s = P.PotentialSolver(w, typ, 20000, 1e-6)
+        s.set_reference_values(0, 1.5, 1e10)
+        ok, res, its = s.solve()
+        assert ok, (typ, res, its)
+        out[typ] = (w.field(P.PHI), its)
+    assert rel_l2(out[P.MG][0], out[P.PCG_SYM][0]) < 1e-8
+    # four Newton steps from phi = 0 at every size here
+    assert out[P.MG][1] <= 4 * 30, out[P.MG][1]
+    assert out[P.MG][1] < out[P.PCG_SYM][1], (out[P.MG][1], out[P.PCG_SYM][1])
