@@ -36,13 +36,14 @@
 namespace pic {
 
 struct MgLevel {
-    Grid g;                   // ni, nj, nk, nn, dh
-    PcgParams Q;              // coefficients at this spacing
-    uint8_t *oid = nullptr;   // object ids (level 0: the world's)
-    uint8_t *mask = nullptr;  // bits 0-5 regular neighbours, bit 6 regular (level 0: the solver's)
-    double *diag = nullptr;   // reduced diagonal (level 0: the solver's jr)
-    double *boltz = nullptr;  // Boltzmann derivative P (level 0: the solver's)
-    double *x = nullptr, *b = nullptr, *r = nullptr;
+    Grid g;                  // ni, nj, nk, nn, dh
+    PcgParams Q;             // coefficients at this spacing
+    uint8_t *oid = nullptr;  // object ids (level 0: the world's)
+    uint8_t *mask = nullptr; // bits 0-5 regular neighbours, bit 6 regular (level 0: the solver's)
+    float *diag = nullptr;   // reduced diagonal
+    float *boltz = nullptr;  // Boltzmann derivative P (levels >= 1)
+    float *x = nullptr, *r = nullptr;
+    float *b = nullptr;      // right-hand side (levels >= 1; level 0 reads the CG residual)
     bool owned = false;
 };
 
@@ -51,22 +52,45 @@ struct MgHierarchy {
     int coarse_sweeps = 0;
 };
 
+// the six neighbour coefficients of a level in slot order (-z, -y, -x, +x, +y, +z)
+struct Coef6 {
+    float c[6];
+};
+
+static Coef6 coef6(const PcgParams &Q) {
+    Coef6 k;
+    k.c[0] = (float)Q.idz2;
+    k.c[1] = (float)Q.idy2;
+    k.c[2] = (float)Q.idx2;
+    k.c[3] = (float)Q.idx2;
+    k.c[4] = (float)Q.idy2;
+    k.c[5] = (float)Q.idz2;
+    return k;
+}
+
 // ---- kernels -----------------------------------------------------------------------------
 // coarse object id and P by injection from the fine node at the same position
+template <typename TP>
 __global__ void __launch_bounds__(256)
-    k_mg_inject(Grid gf, Grid gc, const uint8_t *oid_f, const double *P_f, uint8_t *oid_c, double *P_c) {
+    k_mg_inject(Grid gf, Grid gc, const uint8_t *oid_f, const TP *P_f, uint8_t *oid_c, float *P_c) {
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= gc.nn) return;
     int i, j, k;
     node_ijk(gc, u, i, j, k);
     long f = 2L * i + (long)gf.ni * (2L * j + (long)gf.nj * 2L * k);
     oid_c[u] = oid_f[f];
-    P_c[u] = P_f[f];
+    P_c[u] = (float)P_f[f];
+}
+
+// level-0 diagonal in FP32
+__global__ void __launch_bounds__(256) k_mg_tof(long n, const double *a, float *b) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u < n) b[u] = (float)a[u];
 }
 
 // mask and reduced diagonal of a coarse level (the operator part of k_nrs_F)
 __global__ void __launch_bounds__(256)
-    k_mg_ops(PcgParams Q, const uint8_t *oid, const double *P, uint8_t *mask, double *diag) {
+    k_mg_ops(PcgParams Q, const uint8_t *oid, const float *P, uint8_t *mask, float *diag) {
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
@@ -77,7 +101,7 @@ __global__ void __launch_bounds__(256)
         diag[u] = 1;
         return;
     }
-    double dg = -2.0 * (Q.idx2 + Q.idy2 + Q.idz2) - P[u];
+    double dg = -2.0 * (Q.idx2 + Q.idy2 + Q.idz2) - (double)P[u];
     uint8_t m = 1u << 6;
     const int di[6] = {0, 0, -1, 1, 0, 0}, dj[6] = {0, -1, 0, 0, 1, 0}, dk[6] = {-1, 0, 0, 0, 0, 1};
     for (int d = 0; d < 6; d++) {
@@ -87,28 +111,28 @@ __global__ void __launch_bounds__(256)
         else if (tv == NT_NEU) dg += nb_coef(Q, d);
     }
     mask[u] = m;
-    diag[u] = dg;
+    diag[u] = (float)dg;
 }
 
 // first half-sweep from x = 0: red regular nodes x = b/diag, every other node 0
+template <typename TB>
 __global__ void __launch_bounds__(256)
-    k_mg_smooth_first(Grid g, const uint8_t *mask, const double *diag, double *x, const double *b,
-                      const DevScalars *sc) {
+    k_mg_smooth_first(Grid g, const uint8_t *mask, const float *diag, float *x, const TB *b, const DevScalars *sc) {
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
     int i, j, k;
     node_ijk(g, u, i, j, k);
     bool red = ((i + j + k) & 1) == 0;
-    x[u] = (red && (mask[u] & 64)) ? b[u] / diag[u] : 0.0;
+    x[u] = (red && (mask[u] & 64)) ? (float)b[u] / diag[u] : 0.0f;
 }
 
 // Gauss-Seidel half-sweep over the regular nodes of one colour
+template <typename TB>
 __global__ void __launch_bounds__(256)
-    k_mg_smooth(PcgParams Q, const uint8_t *mask, const double *diag, double *x, const double *b, int color,
+    k_mg_smooth(Grid g, Coef6 C, const uint8_t *mask, const float *diag, float *x, const TB *b, int color,
                 const DevScalars *sc) {
     if (sc->done) return;
-    const Grid &g = Q.g;
     // thread t handles the t-th node of this colour: x-fastest, every second node of a row
     long half = (g.ni + 1) / 2;
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -120,45 +144,53 @@ __global__ void __launch_bounds__(256)
     long u = i + (long)g.ni * row;
     uint8_t m = mask[u];
     if (!(m & 64)) return;
-    double s = 0;
+    float s = 0;
     for (int d = 0; d < 6; d++)
-        if (m & (1u << d)) s += nb_coef(Q, d) * x[u + nb_offset(g, d)];
-    x[u] = (b[u] - s) / diag[u];
+        if (m & (1u << d)) s += C.c[d] * x[u + nb_offset(g, d)];
+    x[u] = ((float)b[u] - s) / diag[u];
 }
 
 // r = b - J x on regular nodes, 0 elsewhere
+template <typename TB>
 __global__ void __launch_bounds__(256)
-    k_mg_residual(PcgParams Q, const uint8_t *mask, const double *diag, const double *x, const double *b,
-                  double *r, const DevScalars *sc) {
+    k_mg_residual(Grid g, Coef6 C, const uint8_t *mask, const float *diag, const float *x, const TB *b, float *r,
+                  const DevScalars *sc) {
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (u >= Q.g.nn) return;
+    if (u >= g.nn) return;
     uint8_t m = mask[u];
-    r[u] = (m & 64) ? b[u] - jr_apply(Q, u, m, diag[u], x) : 0.0;
+    float v = 0.0f;
+    if (m & 64) {
+        float s = diag[u] * x[u];
+        for (int d = 0; d < 6; d++)
+            if (m & (1u << d)) s += C.c[d] * x[u + nb_offset(g, d)];
+        v = (float)b[u] - s;
+    }
+    r[u] = v;
 }
 
 // full weighting of the fine residual onto every coarse node (faces included)
 __global__ void __launch_bounds__(256)
-    k_mg_restrict(Grid gf, Grid gc, const double *rf, double *rc, const DevScalars *sc) {
+    k_mg_restrict(Grid gf, Grid gc, const float *rf, float *rc, const DevScalars *sc) {
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= gc.nn) return;
     int I, J, K;
     node_ijk(gc, u, I, J, K);
     long sj = gf.ni, sk = (long)gf.ni * gf.nj;
-    double s = 0;
+    float s = 0;
     for (int c = -1; c <= 1; c++) {
         int k = 2 * K + c;
         if (k < 0 || k >= gf.nk) continue;
-        double wc = c == 0 ? 0.5 : 0.25;
+        float wc = c == 0 ? 0.5f : 0.25f;
         for (int b = -1; b <= 1; b++) {
             int j = 2 * J + b;
             if (j < 0 || j >= gf.nj) continue;
-            double wb = wc * (b == 0 ? 0.5 : 0.25);
+            float wb = wc * (b == 0 ? 0.5f : 0.25f);
             for (int a = -1; a <= 1; a++) {
                 int i = 2 * I + a;
                 if (i < 0 || i >= gf.ni) continue;
-                s += wb * (a == 0 ? 0.5 : 0.25) * rf[i + sj * j + sk * k];
+                s += wb * (a == 0 ? 0.5f : 0.25f) * rf[i + sj * j + sk * k];
             }
         }
     }
@@ -168,18 +200,17 @@ __global__ void __launch_bounds__(256)
 // E^T: every regular coarse node adds the values of the face nodes whose copy chain ends on
 // it (at most 7, fixed order); non-regular nodes become 0
 __global__ void __launch_bounds__(256)
-    k_mg_fold(Grid g, const uint8_t *oid, const uint8_t *mask, const double *in, double *out,
-              const DevScalars *sc) {
+    k_mg_fold(Grid g, const uint8_t *oid, const uint8_t *mask, const float *in, float *out, const DevScalars *sc) {
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
     if (!(mask[u] & 64)) {
-        out[u] = 0.0;
+        out[u] = 0.0f;
         return;
     }
     int i, j, k;
     node_ijk(g, u, i, j, k);
-    double s = in[u];
+    float s = in[u];
     int da0 = i == 1 ? -1 : 0, da1 = i == g.ni - 2 ? 1 : 0;
     int db0 = j == 1 ? -1 : 0, db1 = j == g.nj - 2 ? 1 : 0;
     int dc0 = k == 1 ? -1 : 0, dc1 = k == g.nk - 2 ? 1 : 0;
@@ -197,7 +228,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // E: face nodes take the value of their chain target (Dirichlet nodes keep 0)
-__global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, double *x, const DevScalars *sc) {
+__global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, float *x, const DevScalars *sc) {
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
@@ -209,7 +240,7 @@ __global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, d
 
 // x_f += trilinear interpolation of x_c, on regular fine nodes
 __global__ void __launch_bounds__(256)
-    k_mg_prolong(Grid gf, Grid gc, const uint8_t *mask_f, const double *xc, double *xf, const DevScalars *sc) {
+    k_mg_prolong(Grid gf, Grid gc, const uint8_t *mask_f, const float *xc, float *xf, const DevScalars *sc) {
     if (sc->done) return;
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= gf.nn) return;
@@ -220,12 +251,12 @@ __global__ void __launch_bounds__(256)
     int ai = i & 1, aj = j & 1, ak = k & 1;
     long sj = gc.ni, sk = (long)gc.ni * gc.nj;
     long b = I + sj * J + sk * K;
-    double s = 0;
+    float s = 0;
     for (int c = 0; c <= ak; c++)
         for (int bb = 0; bb <= aj; bb++)
             for (int a = 0; a <= ai; a++) s += xc[b + a + sj * bb + sk * c];
     // weights: 1, 1/2, 1/4 or 1/8 for 1, 2, 4 or 8 coarse nodes
-    xf[u] += s * (1.0 / (double)(1 << (ai + aj + ak)));
+    xf[u] += s * (1.0f / (float)(1 << (ai + aj + ak)));
 }
 
 // ---- host --------------------------------------------------------------------------------
@@ -249,19 +280,20 @@ static unsigned smooth_blocks(const Grid &g) {
 void mg_destroy(MgHierarchy *H) {
     if (!H) return;
     for (auto &l : H->L) {
+        hipFree(l.diag);
+        hipFree(l.x);
+        hipFree(l.r);
         if (!l.owned) continue;
         hipFree(l.oid);
         hipFree(l.mask);
-        hipFree(l.diag);
         hipFree(l.boltz);
-        hipFree(l.x);
         hipFree(l.b);
-        hipFree(l.r);
     }
     delete H;
 }
 
-// Builds the level list (once per solver); level 0 borrows the solver's arrays.
+// Builds the level list (once per solver).  Level 0 borrows the world's object ids and the
+// solver's mask; its right-hand side is the CG residual (FP64) and its solution s->sv.
 int mg_build(pic_solver *s, MgHierarchy **out) {
     if (*out) return PIC_OK;
     pic_world *w = s->w;
@@ -270,11 +302,6 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
     l0.g = w->g;
     l0.oid = w->object_id;
     l0.mask = s->mask;
-    l0.diag = s->P;
-    l0.boltz = s->boltz;
-    l0.x = s->sv;
-    l0.b = s->g;
-    l0.r = s->rv;
     H->L.push_back(l0);
     for (;;) {
         const Grid &f = H->L.back().g;
@@ -287,20 +314,24 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
         c.g.nn = (long)c.g.ni * c.g.nj * c.g.nk;
         for (int a = 0; a < 3; a++) c.g.dh[a] = 2 * f.dh[a];
         c.owned = true;
+        H->L.push_back(c);
+    }
+    hipError_t e = hipSuccess;
+    for (size_t l = 0; l < H->L.size() && e == hipSuccess; l++) {
+        MgLevel &c = H->L[l];
         size_t n = (size_t)c.g.nn;
-        hipError_t e = hipSuccess;
+        if (e == hipSuccess) e = hipMalloc(&c.diag, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&c.x, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&c.r, n * sizeof(float));
+        if (!c.owned) continue;
         if (e == hipSuccess) e = hipMalloc(&c.oid, n);
         if (e == hipSuccess) e = hipMalloc(&c.mask, n);
-        if (e == hipSuccess) e = hipMalloc(&c.diag, n * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&c.boltz, n * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&c.x, n * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&c.b, n * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&c.r, n * sizeof(double));
-        H->L.push_back(c);
-        if (e != hipSuccess) {
-            mg_destroy(H);
-            return hip_fail(e, "hipMalloc (multigrid level)", __FILE__, __LINE__);
-        }
+        if (e == hipSuccess) e = hipMalloc(&c.boltz, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&c.b, n * sizeof(float));
+    }
+    if (e != hipSuccess) {
+        mg_destroy(H);
+        return hip_fail(e, "hipMalloc (multigrid level)", __FILE__, __LINE__);
     }
     const Grid &cg = H->L.back().g;
     H->coarse_sweeps = std::max(8, cg.ni + cg.nj + cg.nk);
@@ -308,53 +339,84 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
     return PIC_OK;
 }
 
-// Coarse-level operators for the current Newton step (level 0's mask, jr and P are fresh
-// from k_nrs_F).
-int mg_setup(MgHierarchy *H, const PcgParams &Q0, hipStream_t st) {
+// Level operators for the current Newton step: level 0's mask, jr and P come fresh from
+// k_nrs_F.
+int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, hipStream_t st) {
     for (size_t l = 0; l < H->L.size(); l++) {
         MgLevel &c = H->L[l];
         c.Q = level_params(Q0, c.g);
-        if (l == 0) continue;
+        if (l == 0) {
+            k_mg_tof<<<nbk(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.diag);
+            continue;
+        }
         const MgLevel &f = H->L[l - 1];
-        k_mg_inject<<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, f.boltz, c.oid, c.boltz);
+        if (l == 1)
+            k_mg_inject<double><<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, boltz0, c.oid, c.boltz);
+        else
+            k_mg_inject<float><<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, f.boltz, c.oid, c.boltz);
         k_mg_ops<<<nbk(c.g.nn), 256, 0, st>>>(c.Q, c.oid, c.boltz, c.mask, c.diag);
     }
     PIC_LAUNCH_CHECK();
     return PIC_OK;
 }
 
-// x_0 = V(b_0): one V-cycle from a zero initial guess on every level
-int mg_vcycle(MgHierarchy *H, hipStream_t st, const DevScalars *sc) {
+// pre-smoothing, residual and restriction of level l (right-hand side b of type TB)
+template <typename TB>
+static void down(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScalars *sc) {
+    MgLevel &f = H->L[l], &c = H->L[l + 1];
+    Coef6 C = coef6(f.Q);
+    k_mg_smooth_first<TB><<<nbk(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b, sc);
+    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 1, sc);
+    k_mg_residual<TB><<<nbk(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, f.r, sc);
+    k_mg_restrict<<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.r, c.r, sc);
+    k_mg_fold<<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
+}
+
+// prolongation of level l+1 into level l, then post-smoothing (black, red)
+template <typename TB>
+static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScalars *sc) {
+    MgLevel &f = H->L[l], &c = H->L[l + 1];
+    Coef6 C = coef6(f.Q);
+    k_mg_extend<<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
+    k_mg_prolong<<<nbk(f.g.nn), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
+    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 1, sc);
+    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 0, sc);
+}
+
+// x0 = V(b0): one V-cycle from a zero initial guess on every level; x0 is level 0's
+// solution (FP32), b0 the CG residual (FP64)
+int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, const DevScalars *sc) {
     const int nl = (int)H->L.size();
-    for (int l = 0; l < nl - 1; l++) {
-        MgLevel &f = H->L[l], &c = H->L[l + 1];
-        k_mg_smooth_first<<<nbk(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, f.b, sc);
-        k_mg_smooth<<<smooth_blocks(f.g), 256, 0, st>>>(f.Q, f.mask, f.diag, f.x, f.b, 1, sc);
-        k_mg_residual<<<nbk(f.g.nn), 256, 0, st>>>(f.Q, f.mask, f.diag, f.x, f.b, f.r, sc);
-        k_mg_restrict<<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.r, c.r, sc);
-        k_mg_fold<<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
+    *x0 = H->L[0].x;
+    if (nl == 1) {  // no coarsening possible: the preconditioner is one symmetric sweep pair
+        MgLevel &f = H->L[0];
+        Coef6 C = coef6(f.Q);
+        k_mg_smooth_first<double><<<nbk(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b0, sc);
+        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 1, sc);
+        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 1, sc);
+        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 0, sc);
+        PIC_LAUNCH_CHECK();
+        return PIC_OK;
     }
+    down<double>(H, 0, b0, st, sc);
+    for (int l = 1; l < nl - 1; l++) down<float>(H, l, H->L[l].b, st, sc);
     {
         MgLevel &c = H->L[nl - 1];
-        k_mg_smooth_first<<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.mask, c.diag, c.x, c.b, sc);
-        k_mg_smooth<<<smooth_blocks(c.g), 256, 0, st>>>(c.Q, c.mask, c.diag, c.x, c.b, 1, sc);
+        Coef6 C = coef6(c.Q);
+        k_mg_smooth_first<float><<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.mask, c.diag, c.x, c.b, sc);
+        k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 1, sc);
         // symmetric sweeps: (red, black) pairs, then as many (black, red) pairs
         for (int it = 1; it < H->coarse_sweeps; it++) {
-            k_mg_smooth<<<smooth_blocks(c.g), 256, 0, st>>>(c.Q, c.mask, c.diag, c.x, c.b, 0, sc);
-            k_mg_smooth<<<smooth_blocks(c.g), 256, 0, st>>>(c.Q, c.mask, c.diag, c.x, c.b, 1, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 0, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 1, sc);
         }
         for (int it = 0; it < H->coarse_sweeps; it++) {
-            k_mg_smooth<<<smooth_blocks(c.g), 256, 0, st>>>(c.Q, c.mask, c.diag, c.x, c.b, 1, sc);
-            k_mg_smooth<<<smooth_blocks(c.g), 256, 0, st>>>(c.Q, c.mask, c.diag, c.x, c.b, 0, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 1, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 0, sc);
         }
     }
-    for (int l = nl - 2; l >= 0; l--) {
-        MgLevel &f = H->L[l], &c = H->L[l + 1];
-        k_mg_extend<<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
-        k_mg_prolong<<<nbk(f.g.nn), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
-        k_mg_smooth<<<smooth_blocks(f.g), 256, 0, st>>>(f.Q, f.mask, f.diag, f.x, f.b, 1, sc);
-        k_mg_smooth<<<smooth_blocks(f.g), 256, 0, st>>>(f.Q, f.mask, f.diag, f.x, f.b, 0, sc);
-    }
+    for (int l = nl - 2; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);
+    up<double>(H, 0, b0, st, sc);
     PIC_LAUNCH_CHECK();
     return PIC_OK;
 }

@@ -66,10 +66,8 @@ struct Sphere {
 struct PartBuf {
     void *mem = nullptr;
     long cap_bins = 0, cap_stage = 0;
-    // PIC_MIXED
-    int *cell = nullptr;
-    float *ox = nullptr, *oy = nullptr, *oz = nullptr;
-    float *vx = nullptr, *vy = nullptr, *vz = nullptr, *w = nullptr;
+    // PIC_MIXED: 32-byte records {int cell (bit pattern), ox, oy, oz | vx, vy, vz, w}
+    float4 *rec = nullptr;
     // PIC_FP64
     double *x = nullptr, *y = nullptr, *z = nullptr;
     double *dvx = nullptr, *dvy = nullptr, *dvz = nullptr, *dw = nullptr;
@@ -85,8 +83,7 @@ struct PartBuf {
 
 // Device view of a buffer, passed by value to kernels.
 struct PartView {
-    int *cell;
-    float *ox, *oy, *oz, *vx, *vy, *vz, *w;
+    float4 *rec;
     double *x, *y, *z, *dvx, *dvy, *dvz, *dw;
     uint32_t *id;
     uint32_t *bin_start, *bin_cap, *bin_off, *bin_own, *bin_in, *stage_count;
@@ -94,7 +91,7 @@ struct PartView {
 };
 
 inline PartView view_of(const PartBuf &b) {
-    return PartView{b.cell, b.ox, b.oy, b.oz, b.vx, b.vy, b.vz, b.w, b.x, b.y, b.z,
+    return PartView{b.rec, b.x, b.y, b.z,
                     b.dvx, b.dvy, b.dvz, b.dw, b.id, b.bin_start, b.bin_cap, b.bin_off,
                     b.bin_own, b.bin_in, b.stage_count, b.cap_bins, b.cap_stage};
 }
@@ -198,8 +195,8 @@ struct pic_solver {
     // (P holds the Jacobian diagonal jd = A(u,u) - P(u); d and z2 are the ping-pong pair of d)
     double *y = nullptr, *g = nullptr, *d = nullptr, *z = nullptr, *F = nullptr, *P = nullptr, *z2 = nullptr;
     uint8_t *mask = nullptr;  // PCG_SYM: regular-neighbour bits per node
-    // PIC_SOLVER_MG: preconditioned residual s, V-cycle scratch, Boltzmann derivative, levels
-    double *sv = nullptr, *rv = nullptr, *boltz = nullptr;
+    // PIC_SOLVER_MG: Boltzmann derivative of the Newton step, multigrid levels
+    double *boltz = nullptr;
     pic::MgHierarchy *mg = nullptr;
 };
 

@@ -676,18 +676,18 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void __launch_bounds__(256)
-    k_mgpcg_dot(long nn, const double *a, const double *b, double *partials, const DevScalars *sc) {
+    k_mgpcg_dot(long nn, const double *a, const float *b, double *partials, const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
     NodeChunk ch = pcg_chunk(nn);
     double s = 0;
-    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) s += a[u] * b[u];
+    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) s += a[u] * (double)b[u];
     double r = block_sum(s, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
 __global__ void __launch_bounds__(256)
-    k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const double *sv, const double *d_old,
+    k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const float *sv, const double *d_old,
               double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
@@ -699,18 +699,18 @@ __global__ void __launch_bounds__(256)
         uint8_t m = mask[u];
         if (m & 64) {
             y[u] = y[u] + lam * d_old[u];
-            double dc = mu * d_old[u] - sv[u];
+            double dc = mu * d_old[u] - (double)sv[u];
             double zz = 0;
             for (int d = 0; d < 3; d++)
                 if (m & (1u << d)) {
                     long v = u + nb_offset(g, d);
-                    zz += nb_coef(Q, d) * (mu * d_old[v] - sv[v]);
+                    zz += nb_coef(Q, d) * (mu * d_old[v] - (double)sv[v]);
                 }
             zz += jr[u] * dc;
             for (int d = 3; d < 6; d++)
                 if (m & (1u << d)) {
                     long v = u + nb_offset(g, d);
-                    zz += nb_coef(Q, d) * (mu * d_old[v] - sv[v]);
+                    zz += nb_coef(Q, d) * (mu * d_old[v] - (double)sv[v]);
                 }
             d_new[u] = dc;
             z[u] = zz;
@@ -1119,8 +1119,8 @@ static int gsls_linear(pic_solver *s, const PcgParams &Q) {
 
 // Newton iteration of solveNRPCG (:225-296) on the Neumann-reduced system
 int mg_build(pic_solver *s, MgHierarchy **out);
-int mg_setup(MgHierarchy *H, const PcgParams &Q0, hipStream_t st);
-int mg_vcycle(MgHierarchy *H, hipStream_t st, const DevScalars *sc);
+int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, hipStream_t st);
+int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, const DevScalars *sc);
 void mg_destroy(MgHierarchy *H);
 
 // solvePCGLinear on the reduced system with one V-cycle as the preconditioner
@@ -1137,9 +1137,10 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     hipStream_t st = w->stream;
     k_reset_flags<<<1, 1, 0, st>>>(w->scal);
     k_mgpcg_init<<<nb, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0]);
-    rc = mg_vcycle(s->mg, st, w->scal);
+    float *sv = nullptr;  // s = V(g), FP32
+    rc = mg_vcycle(s->mg, s->g, &sv, st, w->scal);
     if (rc) return rc;
-    k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, s->sv, w->partials, w->scal);
+    k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, sv, w->partials, w->scal);
     k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 0, nu, Q.tol, 0, w->scal);
     // a few iterations are queued between host checks of the convergence flag; kernels of
     // iterations past convergence return at once
@@ -1148,15 +1149,15 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_mgpcg_a<<<nb, 256, 0, st>>>(Q, s->mask, s->P, s->sv, dbuf[it & 1], dbuf[(it + 1) & 1], s->y, s->z,
+            k_mgpcg_a<<<nb, 256, 0, st>>>(Q, s->mask, s->P, sv, dbuf[it & 1], dbuf[(it + 1) & 1], s->y, s->z,
                                          w->partials, w->scal);
             k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, nb, folded, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_mgpcg_b<<<G, 256, 0, st>>>(nn, s->g, s->z, w->partials, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
-            rc = mg_vcycle(s->mg, st, w->scal);
+            rc = mg_vcycle(s->mg, s->g, &sv, st, w->scal);
             if (rc) return rc;
-            k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, s->sv, w->partials, w->scal);
+            k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, sv, w->partials, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 3, nu, Q.tol, it, w->scal);
         }
         PIC_LAUNCH_CHECK();
@@ -1176,12 +1177,7 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
 }
 
 static int ensure_mg(pic_solver *s) {
-    if (!s->sv) {
-        size_t n = (size_t)s->w->g.nn;
-        PIC_HIP(hipMalloc(&s->sv, n * sizeof(double)));
-        PIC_HIP(hipMalloc(&s->rv, n * sizeof(double)));
-        PIC_HIP(hipMalloc(&s->boltz, n * sizeof(double)));
-    }
+    if (!s->boltz) PIC_HIP(hipMalloc(&s->boltz, (size_t)s->w->g.nn * sizeof(double)));
     return mg_build(s, &s->mg);
 }
 
@@ -1215,7 +1211,7 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
         PIC_LAUNCH_CHECK();
         bool pc = false;
         if (use_mg) {
-            rc = mg_setup(s->mg, Q, w->stream);
+            rc = mg_setup(s->mg, Q, s->P, s->boltz, w->stream);
             if (rc) return rc;
             rc = mgpcg_linear(s, Q, &pc, &pcg_its);
         } else {
@@ -1280,8 +1276,6 @@ int pic_solver_destroy(pic_solver *s) {
     hipFree(s->z2);
     hipFree(s->mask);
     mg_destroy(s->mg);
-    hipFree(s->sv);
-    hipFree(s->rv);
     hipFree(s->boltz);
     delete s;
     return PIC_OK;
