@@ -34,8 +34,13 @@ QE = 1.602176565e-19
 AMU = 1.660538921e-27
 K_B = 1.380648e-23
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-BYTES_PER_PARTICLE_STEP = 64  # read 32 (cell, 3 offsets, 3 v, w) + write 32 (rebinned copy)
-BYTES_PER_NODE_E = 12         # FP32 E read per node
+# Algorithmic bytes of one push+gather+deposit launch: a particle must be read (cell, 3
+# offsets, 3 velocities, weight: 32 B) and its new cell, offsets and velocities written
+# (28 B; the weight never changes), and E read once per node (FP32, 12 B).  The kernel
+# itself writes 32 B per particle, weight included, since every particle moves to a new
+# slot each step.
+BYTES_PER_PARTICLE_STEP = 60
+BYTES_PER_NODE_E = 12
 
 
 def parse():
@@ -96,6 +101,26 @@ def push_only(P, args, grid):
                       "deposit_avg_ms": tm["deposit"][0] / tm["deposit"][1],
                       "calibration": {"kernel": "k_avg", "read_bytes": 16 * ni * nj * nk,
                                       "write_bytes": 8 * ni * nj * nk}}), flush=True)
+
+
+def copy_bandwidth(nbytes=1 << 31, reps=10):
+    """Achievable HBM bandwidth on this box: a plain elementwise copy kernel (torch) over
+    nbytes, read + write counted, timed with events.  Reported beside the roofline peak."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.mul(a, 1.0, out=b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    gbps = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbps
 
 
 def log(*a):
@@ -288,16 +313,19 @@ def main():
     np_avg = mixed["particle_steps"] / steps
     bytes_launch = np_avg * BYTES_PER_PARTICLE_STEP + mixed["nn"] * BYTES_PER_NODE_E
     achieved = bytes_launch / (mixed["push_avg_ms"] * 1e-3)
+    # measured HBM bytes of the push (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated; see
+    # profiles/push_sphere257_traffic.md) scaled to this run's particle count
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "push_mixed_sphere257_traffic.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and tuple(grid) == (257, 257, 513):
         try:
-            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
-        except Exception:
+            traffic = json.load(open(tfile))["hbm_bytes_per_particle"] * np_avg
+        except (OSError, ValueError, KeyError):
             traffic = None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "kernel": "k_push<PIC_MIXED>", "bytes_per_launch": bytes_launch}
+                "kernel": "k_push<PIC_MIXED>", "bytes_per_launch": bytes_launch,
+                "copy_kernel_gbps": copy_bandwidth()}
 
     cpu = None
     if not args.no_cpu:

@@ -73,7 +73,7 @@ static Coef6 coef6(const PcgParams &Q) {
 template <typename TP>
 __global__ void __launch_bounds__(256)
     k_mg_inject(Grid gf, Grid gc, const uint8_t *oid_f, const TP *P_f, uint8_t *oid_c, float *P_c) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= gc.nn) return;
     int i, j, k;
     node_ijk(gc, u, i, j, k);
@@ -84,14 +84,14 @@ __global__ void __launch_bounds__(256)
 
 // level-0 diagonal in FP32
 __global__ void __launch_bounds__(256) k_mg_tof(long n, const double *a, float *b) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u < n) b[u] = (float)a[u];
 }
 
 // mask and reduced diagonal of a coarse level (the operator part of k_nrs_F)
 __global__ void __launch_bounds__(256)
     k_mg_ops(PcgParams Q, const uint8_t *oid, const float *P, uint8_t *mask, float *diag) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
     int i, j, k;
@@ -119,7 +119,7 @@ template <typename TB>
 __global__ void __launch_bounds__(256)
     k_mg_smooth_first(Grid g, const uint8_t *mask, const float *diag, float *x, const TB *b, const DevScalars *sc) {
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
     int i, j, k;
     node_ijk(g, u, i, j, k);
@@ -134,12 +134,13 @@ __global__ void __launch_bounds__(256)
                 const DevScalars *sc) {
     if (sc->done) return;
     // thread t handles the t-th node of this colour: x-fastest, every second node of a row
-    long half = (g.ni + 1) / 2;
-    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    long row = t / half;
-    if (row >= (long)g.nj * g.nk) return;
-    int j = (int)(row % g.nj), k = (int)(row / g.nj);
-    int i = (int)(2 * (t % half)) + ((j + k + color) & 1);
+    unsigned half = (unsigned)(g.ni + 1) / 2;
+    long t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= (long)half * g.nj * g.nk) return;
+    unsigned row = (unsigned)t / half, ih = (unsigned)t - row * half;
+    unsigned kk = row / (unsigned)g.nj;
+    int j = (int)(row - kk * (unsigned)g.nj), k = (int)kk;
+    int i = (int)(2 * ih) + ((j + k + color) & 1);
     if (i >= g.ni) return;
     long u = i + (long)g.ni * row;
     uint8_t m = mask[u];
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(256)
     k_mg_residual(Grid g, Coef6 C, const uint8_t *mask, const float *diag, const float *x, const TB *b, float *r,
                   const DevScalars *sc) {
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
     uint8_t m = mask[u];
     float v = 0.0f;
@@ -173,7 +174,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_mg_restrict(Grid gf, Grid gc, const float *rf, float *rc, const DevScalars *sc) {
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= gc.nn) return;
     int I, J, K;
     node_ijk(gc, u, I, J, K);
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_mg_fold(Grid g, const uint8_t *oid, const uint8_t *mask, const float *in, float *out, const DevScalars *sc) {
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
     if (!(mask[u] & 64)) {
         out[u] = 0.0f;
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(256)
 // E: face nodes take the value of their chain target (Dirichlet nodes keep 0)
 __global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, float *x, const DevScalars *sc) {
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
     int i, j, k;
     node_ijk(g, u, i, j, k);
@@ -242,7 +243,7 @@ __global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, f
 __global__ void __launch_bounds__(256)
     k_mg_prolong(Grid gf, Grid gc, const uint8_t *mask_f, const float *xc, float *xf, const DevScalars *sc) {
     if (sc->done) return;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= gf.nn) return;
     if (!(mask_f[u] & 64)) return;
     int i, j, k;
@@ -274,7 +275,7 @@ static PcgParams level_params(const PcgParams &Q0, const Grid &g) {
 
 static unsigned smooth_blocks(const Grid &g) {
     long half = (g.ni + 1) / 2;
-    return nbk(half * (long)g.nj * g.nk);
+    return nbk8(half * (long)g.nj * g.nk);
 }
 
 void mg_destroy(MgHierarchy *H) {
@@ -346,15 +347,15 @@ int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const doubl
         MgLevel &c = H->L[l];
         c.Q = level_params(Q0, c.g);
         if (l == 0) {
-            k_mg_tof<<<nbk(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.diag);
+            k_mg_tof<<<nbk8(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.diag);
             continue;
         }
         const MgLevel &f = H->L[l - 1];
         if (l == 1)
-            k_mg_inject<double><<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, boltz0, c.oid, c.boltz);
+            k_mg_inject<double><<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, boltz0, c.oid, c.boltz);
         else
-            k_mg_inject<float><<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, f.boltz, c.oid, c.boltz);
-        k_mg_ops<<<nbk(c.g.nn), 256, 0, st>>>(c.Q, c.oid, c.boltz, c.mask, c.diag);
+            k_mg_inject<float><<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, f.boltz, c.oid, c.boltz);
+        k_mg_ops<<<nbk8(c.g.nn), 256, 0, st>>>(c.Q, c.oid, c.boltz, c.mask, c.diag);
     }
     PIC_LAUNCH_CHECK();
     return PIC_OK;
@@ -365,11 +366,11 @@ template <typename TB>
 static void down(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScalars *sc) {
     MgLevel &f = H->L[l], &c = H->L[l + 1];
     Coef6 C = coef6(f.Q);
-    k_mg_smooth_first<TB><<<nbk(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b, sc);
+    k_mg_smooth_first<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b, sc);
     k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 1, sc);
-    k_mg_residual<TB><<<nbk(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, f.r, sc);
-    k_mg_restrict<<<nbk(c.g.nn), 256, 0, st>>>(f.g, c.g, f.r, c.r, sc);
-    k_mg_fold<<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
+    k_mg_residual<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, f.r, sc);
+    k_mg_restrict<<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.r, c.r, sc);
+    k_mg_fold<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
 }
 
 // prolongation of level l+1 into level l, then post-smoothing (black, red)
@@ -377,8 +378,8 @@ template <typename TB>
 static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScalars *sc) {
     MgLevel &f = H->L[l], &c = H->L[l + 1];
     Coef6 C = coef6(f.Q);
-    k_mg_extend<<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
-    k_mg_prolong<<<nbk(f.g.nn), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
+    k_mg_extend<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
+    k_mg_prolong<<<nbk8(f.g.nn), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
     k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 1, sc);
     k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 0, sc);
 }
@@ -391,7 +392,7 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, cons
     if (nl == 1) {  // no coarsening possible: the preconditioner is one symmetric sweep pair
         MgLevel &f = H->L[0];
         Coef6 C = coef6(f.Q);
-        k_mg_smooth_first<double><<<nbk(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b0, sc);
+        k_mg_smooth_first<double><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b0, sc);
         k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 1, sc);
         k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 1, sc);
         k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 0, sc);
@@ -403,7 +404,7 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, cons
     {
         MgLevel &c = H->L[nl - 1];
         Coef6 C = coef6(c.Q);
-        k_mg_smooth_first<float><<<nbk(c.g.nn), 256, 0, st>>>(c.g, c.mask, c.diag, c.x, c.b, sc);
+        k_mg_smooth_first<float><<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.mask, c.diag, c.x, c.b, sc);
         k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 1, sc);
         // symmetric sweeps: (red, black) pairs, then as many (black, red) pairs
         for (int it = 1; it < H->coarse_sweeps; it++) {

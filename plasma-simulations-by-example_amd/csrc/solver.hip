@@ -267,7 +267,7 @@ __device__ inline double row_apply(const PcgParams &Q, int type, long u, int i, 
 // jd = A(u,u) - P(u), P = d(ne term)/dphi (ch3/ver2/PotentialSolver.cpp:240-266)
 __global__ void __launch_bounds__(256)
     k_nr_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *jd) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
     int i, j, k;
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(256)
     __shared__ double sh[4];
     if (sc->done) return;
     const Grid &g = Q.g;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
     if (u < g.nn) {
@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(256)
 
 // the deferred y update of the last completed iteration
 __global__ void __launch_bounds__(256) k_pcg_finish(long nn, const double *d, double *y, const DevScalars *sc) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     double lam = sc->red[7];
     if (u < nn) y[u] = y[u] + lam * d[u];
 }
@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_nr_update(PcgParams Q, const uint8_t *oid, double *x, double *y, double *partials) {
     __shared__ double sh[4];
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     double y2 = 0;
     if (u < g.nn) {
@@ -552,7 +552,7 @@ __global__ void k_fill(long n, double *p, double v) {
 
 // x on face nodes := x at their chain target
 __global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid, double *x) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
     int i, j, k;
     node_ijk(g, u, i, j, k);
@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid
 __global__ void __launch_bounds__(256)
     k_nrs_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *jr,
             uint8_t *mask, double *boltz) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
     int i, j, k;
@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(256)
     __shared__ double sh[4];
     if (sc->done) return;
     const Grid &g = Q.g;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
     if (u < g.nn) {
@@ -668,7 +668,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_mgpcg_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
                  double *d0) {
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= Q.g.nn) return;
     uint8_t m = mask[u];
     gv[u] = (m & 64) ? jr_apply(Q, u, m, jr[u], y) - F[u] : 0.0;
@@ -692,7 +692,7 @@ __global__ void __launch_bounds__(256)
     __shared__ double sh[4];
     if (sc->done) return;
     const Grid &g = Q.g;
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
     if (u < g.nn) {
@@ -807,7 +807,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_nrs_update(PcgParams Q, const uint8_t *oid, double *x, double *y, double *partials) {
     __shared__ double sh[4];
-    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     double y2 = 0;
     if (u < g.nn) {
@@ -931,8 +931,9 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
+    unsigned nb8 = nbk8(nn);  // stencil kernels: XCD-remapped grid, one partial per workgroup
     unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // streaming-pass workgroups = partials
-    int rc = grow_partials(w, (int)nb);
+    int rc = grow_partials(w, (int)nb8);
     if (rc) return rc;
     double *folded = w->partials + 4 * (size_t)w->partials_cap;
     double nu = (double)(int)nn;
@@ -945,9 +946,9 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_pcg_a<<<nb, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+            k_pcg_a<<<nb8, 256, 0, w->stream>>>(Q, w->object_id, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
                                               s->z, w->partials, w->scal);
-            k_fold<<<FOLD_GRID, 256, 0, w->stream>>>(w->partials, nb, folded, w->scal);
+            k_fold<<<FOLD_GRID, 256, 0, w->stream>>>(w->partials, nb8, folded, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_pcg_b<<<G, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)G, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
@@ -962,7 +963,7 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     }
     // d of the last completed iteration sits in dbuf[iters & 1]
     int done_its = w->scal_host->iters;
-    k_pcg_finish<<<nb, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    k_pcg_finish<<<nb8, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
     PIC_LAUNCH_CHECK();
     *its += done_its;
     *converged = conv;
@@ -1008,7 +1009,8 @@ static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
     PcgParams Q = pcg_params(s);
     long nn = w->g.nn;
     unsigned nb = nbk(nn);
-    rc = grow_partials(w, (int)nb);
+    unsigned nb8 = nbk8(nn);  // stencil kernels: XCD-remapped grid, one partial per workgroup
+    rc = grow_partials(w, (int)nb8);
     if (rc) return rc;
     // y starts at zero for each solve and is warm-started across Newton steps (:232)
     k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
@@ -1022,7 +1024,7 @@ static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
     double norm = 0;
     int pcg_its = 0;
     for (int it = 0; it < NR_MAX_IT; it++) {
-        k_nr_F<<<nb, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P);
+        k_nr_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P);
         PIC_LAUNCH_CHECK();
         bool pc = false;
         rc = pcg_linear(s, Q, &pc, &pcg_its);
@@ -1031,8 +1033,8 @@ static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
             rc = gs_linear(s, Q);
             if (rc) return rc;
         }
-        k_nr_update<<<nb, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
-        k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, &w->scal->red[5]);
+        k_nr_update<<<nb8, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
+        k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb8, &w->scal->red[5]);
         PIC_LAUNCH_CHECK();
         rc = read_scalars(w);
         if (rc) return rc;
@@ -1056,8 +1058,9 @@ static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
+    unsigned nb8 = nbk8(nn);  // stencil kernels: XCD-remapped grid, one partial per workgroup
     unsigned G = nb < PCG_GRID ? nb : PCG_GRID;  // streaming-pass workgroups = partials
-    int rc = grow_partials(w, (int)nb);
+    int rc = grow_partials(w, (int)nb8);
     if (rc) return rc;
     double *folded = w->partials + 4 * (size_t)w->partials_cap;
     double nu = (double)(int)nn;
@@ -1070,9 +1073,9 @@ static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_pcgs_a<<<nb, 256, 0, w->stream>>>(Q, s->mask, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+            k_pcgs_a<<<nb8, 256, 0, w->stream>>>(Q, s->mask, s->P, s->g, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
                                                s->z, w->partials, w->scal);
-            k_fold<<<FOLD_GRID, 256, 0, w->stream>>>(w->partials, nb, folded, w->scal);
+            k_fold<<<FOLD_GRID, 256, 0, w->stream>>>(w->partials, nb8, folded, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_pcg_b<<<G, 256, 0, w->stream>>>(Q, s->P, s->g, s->z, w->partials, (int)G, w->scal);
             k_pcg_scalar<<<1, 1024, 0, w->stream>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
@@ -1086,7 +1089,7 @@ static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *
         }
     }
     int done_its = w->scal_host->iters;
-    k_pcg_finish<<<nb, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    k_pcg_finish<<<nb8, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
     PIC_LAUNCH_CHECK();
     *its += done_its;
     *converged = conv;
@@ -1128,15 +1131,16 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
+    unsigned nb8 = nbk8(nn);  // stencil kernels: XCD-remapped grid, one partial per workgroup
     unsigned G = nb < PCG_GRID ? nb : PCG_GRID;
-    int rc = grow_partials(w, (int)nb);
+    int rc = grow_partials(w, (int)nb8);
     if (rc) return rc;
     double *folded = w->partials + 4 * (size_t)w->partials_cap;
     double nu = (double)(int)nn;
     double *dbuf[2] = {s->d, s->z2};
     hipStream_t st = w->stream;
     k_reset_flags<<<1, 1, 0, st>>>(w->scal);
-    k_mgpcg_init<<<nb, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0]);
+    k_mgpcg_init<<<nb8, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0]);
     float *sv = nullptr;  // s = V(g), FP32
     rc = mg_vcycle(s->mg, s->g, &sv, st, w->scal);
     if (rc) return rc;
@@ -1149,9 +1153,9 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_mgpcg_a<<<nb, 256, 0, st>>>(Q, s->mask, s->P, sv, dbuf[it & 1], dbuf[(it + 1) & 1], s->y, s->z,
+            k_mgpcg_a<<<nb8, 256, 0, st>>>(Q, s->mask, s->P, sv, dbuf[it & 1], dbuf[(it + 1) & 1], s->y, s->z,
                                          w->partials, w->scal);
-            k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, nb, folded, w->scal);
+            k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, nb8, folded, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_mgpcg_b<<<G, 256, 0, st>>>(nn, s->g, s->z, w->partials, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
@@ -1169,7 +1173,7 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
         }
     }
     int done_its = w->scal_host->iters;
-    k_pcg_finish<<<nb, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    k_pcg_finish<<<nb8, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
     PIC_LAUNCH_CHECK();
     *its += done_its;
     *converged = conv;
@@ -1194,12 +1198,13 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     PcgParams Q = pcg_params(s);
     long nn = w->g.nn;
     unsigned nb = nbk(nn);
-    rc = grow_partials(w, (int)nb);
+    unsigned nb8 = nbk8(nn);  // stencil kernels: XCD-remapped grid, one partial per workgroup
+    rc = grow_partials(w, (int)nb8);
     if (rc) return rc;
     const char *venv = getenv("PIC_VERBOSE");
     bool verbose = venv && venv[0] == '1';
     auto t0 = std::chrono::steady_clock::now();
-    k_neumann_copy<<<nb, 256, 0, w->stream>>>(w->g, w->object_id, w->phi);
+    k_neumann_copy<<<nb8, 256, 0, w->stream>>>(w->g, w->object_id, w->phi);
     k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
     const int NR_MAX_IT = 20;
     const double NR_TOL = 1e-3;
@@ -1207,7 +1212,7 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     double norm = 0;
     int pcg_its = 0;
     for (int it = 0; it < NR_MAX_IT; it++) {
-        k_nrs_F<<<nb, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz);
+        k_nrs_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz);
         PIC_LAUNCH_CHECK();
         bool pc = false;
         if (use_mg) {
@@ -1222,8 +1227,8 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
             rc = gsls_linear(s, Q);
             if (rc) return rc;
         }
-        k_nrs_update<<<nb, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
-        k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb, &w->scal->red[5]);
+        k_nrs_update<<<nb8, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
+        k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb8, &w->scal->red[5]);
         PIC_LAUNCH_CHECK();
         rc = read_scalars(w);
         if (rc) return rc;
