@@ -103,24 +103,34 @@ def push_only(P, args, grid):
                                       "write_bytes": 8 * ni * nj * nk}}), flush=True)
 
 
-def copy_bandwidth(nbytes=1 << 31, reps=10):
+_COPY_PROBE = """
+import torch
+nbytes, reps = 1 << 31, 10
+a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+b = torch.empty_like(a)
+for _ in range(2):
+    torch.mul(a, 1.0, out=b)
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+e0.record()
+for _ in range(reps):
+    torch.mul(a, 1.0, out=b)
+e1.record()
+torch.cuda.synchronize()
+print(2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+"""
+
+
+def copy_bandwidth():
     """Achievable HBM bandwidth on this box: a plain elementwise copy kernel (torch) over
-    nbytes, read + write counted, timed with events.  Reported beside the roofline peak."""
-    import torch
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(2):
-        torch.mul(a, 1.0, out=b)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        torch.mul(a, 1.0, out=b)
-    e1.record()
-    torch.cuda.synchronize()
-    gbps = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del a, b
-    torch.cuda.empty_cache()
-    return gbps
+    2 GiB, read + write counted, timed with events; reported beside the roofline peak.  It
+    runs in a child process: torch ships its own HIP runtime, and two runtimes in one
+    process do not share the device reliably."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, "-c", _COPY_PROBE], capture_output=True, text=True, timeout=300)
+        return float(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (subprocess.TimeoutExpired, ValueError, IndexError):
+        return None
 
 
 def log(*a):
@@ -286,6 +296,7 @@ def main():
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus "
               f"{args.gpus}. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    copy_gbps = None if args.push_only else copy_bandwidth()
     import pic_amd as P
     if P.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -325,7 +336,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "kernel": "k_push<PIC_MIXED>", "bytes_per_launch": bytes_launch,
-                "copy_kernel_gbps": copy_bandwidth()}
+                "copy_kernel_gbps": copy_gbps}
 
     cpu = None
     if not args.no_cpu:

@@ -47,9 +47,13 @@ struct MgLevel {
     bool owned = false;
 };
 
+struct MgDev;
+
 struct MgHierarchy {
     std::vector<MgLevel> L;
     int coarse_sweeps = 0;
+    int fuse_l0 = -1;       // first level of the single-workgroup coarse cycle (-1: none)
+    MgDev *dev = nullptr;   // device copy of the level descriptors
 };
 
 // the six neighbour coefficients of a level in slot order (-z, -y, -x, +x, +y, +z)
@@ -114,68 +118,70 @@ __global__ void __launch_bounds__(256)
     diag[u] = (float)dg;
 }
 
+// ---- per-node steps (shared by the per-level kernels and the fused coarse cycle) ----------
 // first half-sweep from x = 0: red regular nodes x = b/diag, every other node 0
 template <typename TB>
-__global__ void __launch_bounds__(256)
-    k_mg_smooth_first(Grid g, const uint8_t *mask, const float *diag, float *x, const TB *b, const DevScalars *sc) {
-    if (sc->done) return;
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= g.nn) return;
+__device__ inline void mg_smooth_first_node(const Grid &g, const uint8_t *mask, const float *diag, float *x,
+                                            const TB *b, long u) {
     int i, j, k;
     node_ijk(g, u, i, j, k);
     bool red = ((i + j + k) & 1) == 0;
     x[u] = (red && (mask[u] & 64)) ? (float)b[u] / diag[u] : 0.0f;
 }
 
-// Gauss-Seidel half-sweep over the regular nodes of one colour
+// Iterates x, r are zero off the regular nodes (smoothers and prolongation write regular
+// nodes only, and the first half-sweep of a V-cycle zeroes the rest), and every regular node
+// is interior.  So the six neighbours of an interior node are loaded unconditionally and
+// summed with their coefficients: the non-regular ones add 0, as in the reduced operator,
+// and the loads do not wait for the mask.
+
+__device__ inline bool interior(const Grid &g, int i, int j, int k) {
+    return i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1 && k > 0 && k < g.nk - 1;
+}
+
+__device__ inline float nb_sum(const Grid &g, const Coef6 &C, const float *x, long u) {
+    long sj = g.ni, sk = (long)g.ni * g.nj;
+    return C.c[0] * x[u - sk] + C.c[1] * x[u - sj] + C.c[2] * x[u - 1] + C.c[3] * x[u + 1] +
+           C.c[4] * x[u + sj] + C.c[5] * x[u + sk];
+}
+
+// Gauss-Seidel update of the t-th node of one colour (x-fastest, every second node of a row)
 template <typename TB>
-__global__ void __launch_bounds__(256)
-    k_mg_smooth(Grid g, Coef6 C, const uint8_t *mask, const float *diag, float *x, const TB *b, int color,
-                const DevScalars *sc) {
-    if (sc->done) return;
-    // thread t handles the t-th node of this colour: x-fastest, every second node of a row
+__device__ inline void mg_smooth_node(const Grid &g, const Coef6 &C, const uint8_t *mask, const float *diag,
+                                      float *x, const TB *b, int color, long t) {
     unsigned half = (unsigned)(g.ni + 1) / 2;
-    long t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= (long)half * g.nj * g.nk) return;
     unsigned row = (unsigned)t / half, ih = (unsigned)t - row * half;
     unsigned kk = row / (unsigned)g.nj;
     int j = (int)(row - kk * (unsigned)g.nj), k = (int)kk;
     int i = (int)(2 * ih) + ((j + k + color) & 1);
-    if (i >= g.ni) return;
+    if (!interior(g, i, j, k)) return;
     long u = i + (long)g.ni * row;
     uint8_t m = mask[u];
-    if (!(m & 64)) return;
-    float s = 0;
-    for (int d = 0; d < 6; d++)
-        if (m & (1u << d)) s += C.c[d] * x[u + nb_offset(g, d)];
-    x[u] = ((float)b[u] - s) / diag[u];
+    float s = nb_sum(g, C, x, u);
+    float bb = (float)b[u], dg = diag[u];
+    if (m & 64) x[u] = (bb - s) / dg;
 }
+
+__device__ inline long smooth_count(const Grid &g) { return (long)((g.ni + 1) / 2) * g.nj * g.nk; }
 
 // r = b - J x on regular nodes, 0 elsewhere
 template <typename TB>
-__global__ void __launch_bounds__(256)
-    k_mg_residual(Grid g, Coef6 C, const uint8_t *mask, const float *diag, const float *x, const TB *b, float *r,
-                  const DevScalars *sc) {
-    if (sc->done) return;
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= g.nn) return;
-    uint8_t m = mask[u];
+__device__ inline void mg_residual_node(const Grid &g, const Coef6 &C, const uint8_t *mask, const float *diag,
+                                        const float *x, const TB *b, float *r, long u) {
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
     float v = 0.0f;
-    if (m & 64) {
-        float s = diag[u] * x[u];
-        for (int d = 0; d < 6; d++)
-            if (m & (1u << d)) s += C.c[d] * x[u + nb_offset(g, d)];
-        v = (float)b[u] - s;
+    if (interior(g, i, j, k)) {
+        uint8_t m = mask[u];
+        float s = diag[u] * x[u] + nb_sum(g, C, x, u);
+        float bb = (float)b[u];
+        if (m & 64) v = bb - s;
     }
     r[u] = v;
 }
 
-// full weighting of the fine residual onto every coarse node (faces included)
-__global__ void __launch_bounds__(256)
-    k_mg_restrict(Grid gf, Grid gc, const float *rf, float *rc, const DevScalars *sc) {
-    if (sc->done) return;
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= gc.nn) return;
+// full weighting of the fine residual onto coarse node u (faces included)
+__device__ inline void mg_restrict_node(const Grid &gf, const Grid &gc, const float *rf, float *rc, long u) {
     int I, J, K;
     node_ijk(gc, u, I, J, K);
     long sj = gf.ni, sk = (long)gf.ni * gf.nj;
@@ -198,13 +204,10 @@ __global__ void __launch_bounds__(256)
     rc[u] = s;
 }
 
-// E^T: every regular coarse node adds the values of the face nodes whose copy chain ends on
-// it (at most 7, fixed order); non-regular nodes become 0
-__global__ void __launch_bounds__(256)
-    k_mg_fold(Grid g, const uint8_t *oid, const uint8_t *mask, const float *in, float *out, const DevScalars *sc) {
-    if (sc->done) return;
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= g.nn) return;
+// E^T: a regular coarse node adds the values of the face nodes whose copy chain ends on it
+// (at most 7, fixed order); non-regular nodes become 0
+__device__ inline void mg_fold_node(const Grid &g, const uint8_t *oid, const uint8_t *mask, const float *in,
+                                    float *out, long u) {
     if (!(mask[u] & 64)) {
         out[u] = 0.0f;
         return;
@@ -228,11 +231,8 @@ __global__ void __launch_bounds__(256)
     out[u] = s;
 }
 
-// E: face nodes take the value of their chain target (Dirichlet nodes keep 0)
-__global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, float *x, const DevScalars *sc) {
-    if (sc->done) return;
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= g.nn) return;
+// E: a face node takes the value of its chain target (Dirichlet nodes keep 0)
+__device__ inline void mg_extend_node(const Grid &g, const uint8_t *oid, float *x, long u) {
     int i, j, k;
     node_ijk(g, u, i, j, k);
     if (node_type(g, oid, u, i, j, k) != NT_NEU) return;
@@ -240,11 +240,8 @@ __global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, f
 }
 
 // x_f += trilinear interpolation of x_c, on regular fine nodes
-__global__ void __launch_bounds__(256)
-    k_mg_prolong(Grid gf, Grid gc, const uint8_t *mask_f, const float *xc, float *xf, const DevScalars *sc) {
-    if (sc->done) return;
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= gf.nn) return;
+__device__ inline void mg_prolong_node(const Grid &gf, const Grid &gc, const uint8_t *mask_f, const float *xc,
+                                       float *xf, long u) {
     if (!(mask_f[u] & 64)) return;
     int i, j, k;
     node_ijk(gf, u, i, j, k);
@@ -258,6 +255,129 @@ __global__ void __launch_bounds__(256)
             for (int a = 0; a <= ai; a++) s += xc[b + a + sj * bb + sk * c];
     // weights: 1, 1/2, 1/4 or 1/8 for 1, 2, 4 or 8 coarse nodes
     xf[u] += s * (1.0f / (float)(1 << (ai + aj + ak)));
+}
+
+// ---- per-level kernels (one node per thread, XCD-remapped workgroups) ---------------------
+template <typename TB>
+__global__ void __launch_bounds__(256)
+    k_mg_smooth_first(Grid g, const uint8_t *mask, const float *diag, float *x, const TB *b, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < g.nn) mg_smooth_first_node(g, mask, diag, x, b, u);
+}
+
+template <typename TB>
+__global__ void __launch_bounds__(256)
+    k_mg_smooth(Grid g, Coef6 C, const uint8_t *mask, const float *diag, float *x, const TB *b, int color,
+                const DevScalars *sc) {
+    if (sc->done) return;
+    long t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t < smooth_count(g)) mg_smooth_node(g, C, mask, diag, x, b, color, t);
+}
+
+template <typename TB>
+__global__ void __launch_bounds__(256)
+    k_mg_residual(Grid g, Coef6 C, const uint8_t *mask, const float *diag, const float *x, const TB *b, float *r,
+                  const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < g.nn) mg_residual_node(g, C, mask, diag, x, b, r, u);
+}
+
+__global__ void __launch_bounds__(256)
+    k_mg_restrict(Grid gf, Grid gc, const float *rf, float *rc, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < gc.nn) mg_restrict_node(gf, gc, rf, rc, u);
+}
+
+__global__ void __launch_bounds__(256)
+    k_mg_fold(Grid g, const uint8_t *oid, const uint8_t *mask, const float *in, float *out, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < g.nn) mg_fold_node(g, oid, mask, in, out, u);
+}
+
+__global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, float *x, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < g.nn) mg_extend_node(g, oid, x, u);
+}
+
+__global__ void __launch_bounds__(256)
+    k_mg_prolong(Grid gf, Grid gc, const uint8_t *mask_f, const float *xc, float *xf, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < gf.nn) mg_prolong_node(gf, gc, mask_f, xc, xf, u);
+}
+
+// ---- the coarse levels in one workgroup -------------------------------------------------
+// Levels l0..nl-1 (each at most MG_FUSE_NODES nodes) run the down leg, the coarsest solve and
+// the up leg of the V-cycle inside one launch of 1024 threads, a workgroup barrier between
+// steps; each step is the same per-node code as the per-level kernels.  Replaces some 70
+// launches of a few microseconds each per V-cycle.
+struct MgDev {
+    Grid g;
+    Coef6 C;
+    const uint8_t *oid, *mask;
+    const float *diag;
+    float *x, *r, *b;
+};
+
+constexpr long MG_FUSE_NODES = 12000;  // about 12 nodes per thread and step
+constexpr int MG_FUSE_THREADS = 1024;
+
+__global__ void __launch_bounds__(MG_FUSE_THREADS)
+    k_mg_coarse(const MgDev *L, int l0, int nl, int sweeps, const DevScalars *sc) {
+    if (sc->done) return;
+    const long T = MG_FUSE_THREADS;
+    for (int l = l0; l < nl - 1; l++) {
+        const MgDev &f = L[l], &c = L[l + 1];
+        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_smooth_first_node(f.g, f.mask, f.diag, f.x, f.b, u);
+        __syncthreads();
+        for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
+            mg_smooth_node(f.g, f.C, f.mask, f.diag, f.x, f.b, 1, t);
+        __syncthreads();
+        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_residual_node(f.g, f.C, f.mask, f.diag, f.x, f.b, f.r, u);
+        __syncthreads();
+        for (long u = threadIdx.x; u < c.g.nn; u += T) mg_restrict_node(f.g, c.g, f.r, c.r, u);
+        __syncthreads();
+        for (long u = threadIdx.x; u < c.g.nn; u += T) mg_fold_node(c.g, c.oid, c.mask, c.r, c.b, u);
+        __syncthreads();
+    }
+    {
+        const MgDev &c = L[nl - 1];
+        auto half_sweep = [&](int color) {
+            for (long t = threadIdx.x; t < smooth_count(c.g); t += T)
+                mg_smooth_node(c.g, c.C, c.mask, c.diag, c.x, c.b, color, t);
+            __syncthreads();
+        };
+        for (long u = threadIdx.x; u < c.g.nn; u += T) mg_smooth_first_node(c.g, c.mask, c.diag, c.x, c.b, u);
+        __syncthreads();
+        half_sweep(1);
+        // (red, black) pairs, then as many (black, red) pairs: a symmetric coarsest solve
+        for (int it = 1; it < sweeps; it++) {
+            half_sweep(0);
+            half_sweep(1);
+        }
+        for (int it = 0; it < sweeps; it++) {
+            half_sweep(1);
+            half_sweep(0);
+        }
+    }
+    for (int l = nl - 2; l >= l0; l--) {
+        const MgDev &f = L[l], &c = L[l + 1];
+        for (long u = threadIdx.x; u < c.g.nn; u += T) mg_extend_node(c.g, c.oid, c.x, u);
+        __syncthreads();
+        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_prolong_node(f.g, c.g, f.mask, c.x, f.x, u);
+        __syncthreads();
+        for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
+            mg_smooth_node(f.g, f.C, f.mask, f.diag, f.x, f.b, 1, t);
+        __syncthreads();
+        for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
+            mg_smooth_node(f.g, f.C, f.mask, f.diag, f.x, f.b, 0, t);
+        __syncthreads();
+    }
 }
 
 // ---- host --------------------------------------------------------------------------------
@@ -280,6 +400,7 @@ static unsigned smooth_blocks(const Grid &g) {
 
 void mg_destroy(MgHierarchy *H) {
     if (!H) return;
+    hipFree(H->dev);
     for (auto &l : H->L) {
         hipFree(l.diag);
         hipFree(l.x);
@@ -335,7 +456,41 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
         return hip_fail(e, "hipMalloc (multigrid level)", __FILE__, __LINE__);
     }
     const Grid &cg = H->L.back().g;
-    H->coarse_sweeps = std::max(8, cg.ni + cg.nj + cg.nk);
+    // symmetric sweep pairs on the coarsest level: about its width, enough for the few
+    // unknowns it has (3 at 3x3x5, 64 at 6x6x6)
+    H->coarse_sweeps = std::max(2, std::max(cg.ni, std::max(cg.nj, cg.nk)));
+    // levels small enough for one workgroup (level 0's right-hand side is the FP64 CG
+    // residual, so the fused cycle starts at level 1 or deeper)
+    const int nl = (int)H->L.size();
+    for (int l = 1; l < nl; l++)
+        if (H->L[l].g.nn <= MG_FUSE_NODES) {
+            H->fuse_l0 = l;
+            break;
+        }
+    if (H->fuse_l0 > 0) {
+        std::vector<MgDev> hd(nl);
+        for (int l = 0; l < nl; l++) {
+            const MgLevel &v = H->L[l];
+            MgDev &d = hd[l];
+            d.g = v.g;
+            for (int a = 0; a < 6; a++) {
+                double id = 1.0 / v.g.dh[a < 3 ? 2 - a : a - 3];  // slots -z, -y, -x, +x, +y, +z
+                d.C.c[a] = (float)(id * id);
+            }
+            d.oid = v.oid;
+            d.mask = v.mask;
+            d.diag = v.diag;
+            d.x = v.x;
+            d.r = v.r;
+            d.b = v.b;
+        }
+        e = hipMalloc(&H->dev, sizeof(MgDev) * nl);
+        if (e == hipSuccess) e = hipMemcpy(H->dev, hd.data(), sizeof(MgDev) * nl, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            mg_destroy(H);
+            return hip_fail(e, "multigrid level descriptors", __FILE__, __LINE__);
+        }
+    }
     *out = H;
     return PIC_OK;
 }
@@ -400,6 +555,15 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, cons
         return PIC_OK;
     }
     down<double>(H, 0, b0, st, sc);
+    if (H->fuse_l0 > 0) {
+        const int l0 = H->fuse_l0;
+        for (int l = 1; l < l0; l++) down<float>(H, l, H->L[l].b, st, sc);
+        k_mg_coarse<<<1, MG_FUSE_THREADS, 0, st>>>(H->dev, l0, nl, H->coarse_sweeps, sc);
+        for (int l = l0 - 1; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);
+        up<double>(H, 0, b0, st, sc);
+        PIC_LAUNCH_CHECK();
+        return PIC_OK;
+    }
     for (int l = 1; l < nl - 1; l++) down<float>(H, l, H->L[l].b, st, sc);
     {
         MgLevel &c = H->L[nl - 1];

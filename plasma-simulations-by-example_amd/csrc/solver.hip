@@ -689,6 +689,8 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const float *sv, const double *d_old,
               double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
+    // d_old and s are zero off the regular nodes and every regular node is interior, so the
+    // six neighbours are read without waiting for the mask (non-regular ones add 0)
     __shared__ double sh[4];
     if (sc->done) return;
     const Grid &g = Q.g;
@@ -696,29 +698,36 @@ __global__ void __launch_bounds__(256)
     double mu = sc->red[6], lam = sc->red[7];
     double dz = 0;
     if (u < g.nn) {
-        uint8_t m = mask[u];
-        if (m & 64) {
-            y[u] = y[u] + lam * d_old[u];
-            double dc = mu * d_old[u] - (double)sv[u];
-            double zz = 0;
-            for (int d = 0; d < 3; d++)
-                if (m & (1u << d)) {
-                    long v = u + nb_offset(g, d);
-                    zz += nb_coef(Q, d) * (mu * d_old[v] - (double)sv[v]);
-                }
-            zz += jr[u] * dc;
-            for (int d = 3; d < 6; d++)
-                if (m & (1u << d)) {
-                    long v = u + nb_offset(g, d);
-                    zz += nb_coef(Q, d) * (mu * d_old[v] - (double)sv[v]);
-                }
-            d_new[u] = dc;
-            z[u] = zz;
-            dz = dc * zz;
-        } else {
-            d_new[u] = 0;
-            z[u] = 0;
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
+        bool in = i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1 && k > 0 && k < g.nk - 1;
+        double dc = 0, zz = 0;
+        if (in) {
+            long sj = g.ni, sk = (long)g.ni * g.nj;
+            uint8_t m = mask[u];
+            double jru = jr[u], dou = d_old[u], yu = y[u];
+            double n0 = mu * d_old[u - sk] - (double)sv[u - sk];
+            double n1 = mu * d_old[u - sj] - (double)sv[u - sj];
+            double n2 = mu * d_old[u - 1] - (double)sv[u - 1];
+            double n3 = mu * d_old[u + 1] - (double)sv[u + 1];
+            double n4 = mu * d_old[u + sj] - (double)sv[u + sj];
+            double n5 = mu * d_old[u + sk] - (double)sv[u + sk];
+            if (m & 64) {
+                y[u] = yu + lam * dou;
+                dc = mu * dou - (double)sv[u];
+                // slot order of the reference rows: -z, -y, -x, centre, +x, +y, +z
+                zz = Q.idz2 * n0;
+                zz += Q.idy2 * n1;
+                zz += Q.idx2 * n2;
+                zz += jru * dc;
+                zz += Q.idx2 * n3;
+                zz += Q.idy2 * n4;
+                zz += Q.idz2 * n5;
+            }
         }
+        d_new[u] = dc;
+        z[u] = zz;
+        dz = dc * zz;
     }
     double r = block_sum(dz, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
