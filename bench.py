@@ -85,10 +85,11 @@ def push_only(P, args, grid):
     for i in range(args.warmup + args.steps):
         if i == args.warmup:
             w.timers(reset=True)
+        t_before = w.timers()["push"][0]
         ions.advance(P.ABSORB, Case.dt)
         ions.compute_number_density()
         w.sync()
-        log(f"[bench] push-only {args.precision} step {i} done")
+        log(f"[bench] push-only {args.precision} step {i}: push {w.timers()['push'][0] - t_before:.3f} ms")
     tm = w.timers(reset=True)
     n = ions.np()
     # calibration of the HBM byte counters: Field_::updateAverage (k_avg) is a plain
@@ -222,12 +223,15 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     pcg = []
     w.sync()
     t0 = time.perf_counter()
+    push_prev = 0.0
     for i in range(steps):
         n_before = ions.np()
         ok, its = step()
         np_sum += n_before
         pcg.append(its)
-        log(f"[bench]   step {i}: cg its {its} converged {ok}")
+        push_now = w.timers()["push"][0]
+        log(f"[bench]   step {i}: cg its {its} converged {ok}, push {push_now - push_prev:.3f} ms")
+        push_prev = push_now
     w.sync()
     wall = time.perf_counter() - t0
     tm = w.timers(reset=True)

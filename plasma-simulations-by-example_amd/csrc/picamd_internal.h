@@ -21,14 +21,17 @@ constexpr double PI_REF = 3.141592653;  // the reference's Const::PI
 // ---- tiling ---------------------------------------------------------------------------
 // Particles live in per-tile bins; a tile is TILE^3 cells.  The push of one tile is one
 // workgroup: its E tile (TILE+1)^3 nodes sits in LDS, and deposits of the pushed
-// particles land in an LDS window of the tile plus HALO cells on every side.
+// particles land in an LDS window of the tile plus HALO cells on every side.  A particle
+// that ends more than HALO cells outside its tile deposits with global atomics instead; in
+// the sphere flow ions accelerated by the sheath outrun a one-cell rim (1.3% of them after
+// seven steps, and growing), a two-cell rim keeps nearly all deposits in LDS.
 constexpr int TILE_SHIFT = 3;
 constexpr int TILE = 1 << TILE_SHIFT;
 constexpr int TN = TILE + 1;              // E-tile nodes per edge
 constexpr int TNODES = TN * TN * TN;      // 729
-constexpr int HALO = 1;
+constexpr int HALO = 2;
 constexpr int WN = TILE + 1 + 2 * HALO;   // window nodes per edge
-constexpr int WNODES = WN * WN * WN;      // 1331
+constexpr int WNODES = WN * WN * WN;      // 2197
 constexpr int PUSH_THREADS = 256;
 constexpr int PUSH_PER_THREAD = 4;        // particles per thread per batch
 constexpr int PUSH_BATCH = PUSH_THREADS * PUSH_PER_THREAD;
