@@ -433,84 +433,140 @@ __device__ inline void deposit_lds(double *sW, int wi, int wj, int wk, T ox, T o
 // ---------------------------------------------------------------------------------------
 // per-step bin plan: capacities of the output bins from the input counts
 // ---------------------------------------------------------------------------------------
-// One workgroup of 1024 threads.  Tile t gets an own segment of exactly n[t] slots (its
-// input count: every particle that stays fits) followed by an inflow segment of
-// max(n/8, nb/4) + 32 slots, nb being the largest face-neighbour population (one step of
-// arrivals).  The inflow room is scaled down uniformly when the buffer is short.  Mode 1
-// (counting sort) sizes bins from a histogram with no inflow segment.
-constexpr int PLAN_THREADS = 1024;
+// Tile t gets an own segment of exactly n[t] slots (its input count: every particle that
+// stays fits) followed by an inflow segment of max(n/8, nb/4) + 32 slots, nb being the
+// largest face-neighbour population (one step of arrivals).  The inflow room is scaled down
+// uniformly when the buffer is short.  Mode 1 (counting sort) sizes bins from a histogram
+// with no inflow segment.  Five small launches over all tiles: counts, totals, capacities,
+// block offsets, starts (a single-workgroup plan was latency-bound at 0.3 ms for 65k tiles).
+constexpr int PLAN_THREADS = 256;
+constexpr int PLAN_SCAN_THREADS = 1024;
+
+// scratch of the plan (per species): per-tile need and slack, per-block sums, alpha
+struct PlanTmp {
+    uint32_t *need, *slack;
+    unsigned long long *blk;  // [2 * nblocks]
+    double *alpha;            // [0] alpha, [1] 1 when the buffer is short
+};
 
 __device__ inline uint32_t tile_count(const PartView &v, int t) {
     BinSeg b = bin_of(v, t);
     return b.c0 + b.c1;
 }
 
-__global__ void __launch_bounds__(PLAN_THREADS)
-    k_plan(Grid g, PartView in, const uint32_t *hist, int mode, PartView out, uint32_t *err) {
-    __shared__ unsigned long long s_need[PLAN_THREADS / 64], s_slack[PLAN_THREADS / 64];
-    __shared__ unsigned long long s_scan[PLAN_THREADS];
-    int nt = g.ntiles;
-    int per = (nt + PLAN_THREADS - 1) / PLAN_THREADS;
-    int t0 = threadIdx.x * per, t1 = min(nt, t0 + per);
-    auto sizes = [&](int t, uint32_t &n, uint32_t &s) {
-        if (mode == 0) {
-            n = tile_count(in, t);
-            uint32_t nb = 0;
-            int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
-            if (tx > 0) nb = max(nb, tile_count(in, t - 1));
-            if (tx < g.ntx - 1) nb = max(nb, tile_count(in, t + 1));
-            if (ty > 0) nb = max(nb, tile_count(in, t - g.ntx));
-            if (ty < g.nty - 1) nb = max(nb, tile_count(in, t + g.ntx));
-            if (tz > 0) nb = max(nb, tile_count(in, t - g.ntx * g.nty));
-            if (tz < g.ntz - 1) nb = max(nb, tile_count(in, t + g.ntx * g.nty));
-            s = max(n / 8, nb / 4) + 32;
-        } else {
-            n = hist[t];
-            s = 0;
-        }
-    };
-    unsigned long long need = 0, slack = 0;
-    for (int t = t0; t < t1; t++) {
-        uint32_t n, s;
-        sizes(t, n, s);
-        need += n;
-        slack += s;
+__device__ inline void plan_sizes(const Grid &g, const PartView &in, const uint32_t *hist, int mode, int t,
+                                  uint32_t &n, uint32_t &s) {
+    if (mode == 0) {
+        n = tile_count(in, t);
+        uint32_t nb = 0;
+        int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+        if (tx > 0) nb = max(nb, tile_count(in, t - 1));
+        if (tx < g.ntx - 1) nb = max(nb, tile_count(in, t + 1));
+        if (ty > 0) nb = max(nb, tile_count(in, t - g.ntx));
+        if (ty < g.nty - 1) nb = max(nb, tile_count(in, t + g.ntx));
+        if (tz > 0) nb = max(nb, tile_count(in, t - g.ntx * g.nty));
+        if (tz < g.ntz - 1) nb = max(nb, tile_count(in, t + g.ntx * g.nty));
+        s = max(n / 8, nb / 4) + 32;
+    } else {
+        n = hist[t];
+        s = 0;
     }
-    unsigned long long a = need, b = slack;
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_down(a, off, 64);
-        b += __shfl_down(b, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        s_need[threadIdx.x >> 6] = a;
-        s_slack[threadIdx.x >> 6] = b;
-    }
+}
+
+__device__ inline unsigned long long block_sum_u64(unsigned long long v, unsigned long long *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
     __syncthreads();
-    unsigned long long tot_need = 0, tot_slack = 0;
-    for (int i = 0; i < PLAN_THREADS / 64; i++) {
-        tot_need += s_need[i];
-        tot_slack += s_slack[i];
+    unsigned long long r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(PLAN_THREADS)
+    k_plan_count(Grid g, PartView in, const uint32_t *hist, int mode, PlanTmp T) {
+    __shared__ unsigned long long sh[PLAN_THREADS / 64];
+    int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
+    uint32_t n = 0, sl = 0;
+    if (t < g.ntiles) {
+        plan_sizes(g, in, hist, mode, t, n, sl);
+        T.need[t] = n;
+        T.slack[t] = sl;
     }
-    const unsigned long long capb = (unsigned long long)out.cap_bins;
-    bool short_buf = tot_need > capb;
-    double alpha = 1.0;
-    if (short_buf) {
-        alpha = 0.0;
-        if (threadIdx.x == 0) atomicOr(err, 1u);  // the caller grows the buffers
-    } else if (tot_need + tot_slack > capb) {
-        alpha = (double)(capb - tot_need) / (double)tot_slack;
+    unsigned long long a = block_sum_u64(n, sh), b = block_sum_u64(sl, sh);
+    if (threadIdx.x == 0) {
+        T.blk[2 * blockIdx.x] = a;
+        T.blk[2 * blockIdx.x + 1] = b;
     }
-    unsigned long long local = 0;
-    for (int t = t0; t < t1; t++) {
-        uint32_t n, s;
-        sizes(t, n, s);
-        uint32_t own = short_buf ? 0u : n;
-        uint32_t c = own + (uint32_t)(alpha * (double)s);
+}
+
+__global__ void __launch_bounds__(PLAN_SCAN_THREADS)
+    k_plan_total(PlanTmp T, int nblk, unsigned long long capb, uint32_t *err) {
+    __shared__ unsigned long long sh[PLAN_SCAN_THREADS / 64];
+    unsigned long long a = 0, b = 0;
+    for (int i = threadIdx.x; i < nblk; i += PLAN_SCAN_THREADS) {
+        a += T.blk[2 * i];
+        b += T.blk[2 * i + 1];
+    }
+    unsigned long long need = block_sum_u64(a, sh), slack = block_sum_u64(b, sh);
+    if (threadIdx.x == 0) {
+        double alpha = 1.0, short_buf = 0.0;
+        if (need > capb) {
+            alpha = 0.0;
+            short_buf = 1.0;
+            atomicOr(err, 1u);  // the caller grows the buffers
+        } else if (need + slack > capb) {
+            alpha = (double)(capb - need) / (double)slack;
+        }
+        T.alpha[0] = alpha;
+        T.alpha[1] = short_buf;
+    }
+}
+
+__global__ void __launch_bounds__(PLAN_THREADS) k_plan_caps(Grid g, PlanTmp T, PartView out) {
+    __shared__ unsigned long long sh[PLAN_THREADS / 64];
+    int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
+    uint32_t c = 0;
+    if (t < g.ntiles) {
+        uint32_t own = T.alpha[1] != 0.0 ? 0u : T.need[t];
+        c = own + (uint32_t)(T.alpha[0] * (double)T.slack[t]);
         out.bin_cap[t] = c;
         out.bin_off[t] = own;
-        local += c;
     }
+    unsigned long long a = block_sum_u64(c, sh);
+    if (threadIdx.x == 0) T.blk[blockIdx.x] = a;
+}
+
+// exclusive scan of the nblk block sums in place (one workgroup, fixed order)
+__global__ void __launch_bounds__(PLAN_SCAN_THREADS) k_plan_scan(PlanTmp T, int nblk) {
+    __shared__ unsigned long long s_scan[PLAN_SCAN_THREADS];
+    int per = (nblk + PLAN_SCAN_THREADS - 1) / PLAN_SCAN_THREADS;
+    int b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
+    unsigned long long local = 0;
+    for (int b = b0; b < b1; b++) local += T.blk[b];
     s_scan[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < PLAN_SCAN_THREADS; off <<= 1) {
+        unsigned long long v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = s_scan[threadIdx.x] - local;
+    for (int b = b0; b < b1; b++) {
+        unsigned long long v = T.blk[b];
+        T.blk[b] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(PLAN_THREADS) k_plan_starts(Grid g, PlanTmp T, PartView out) {
+    __shared__ unsigned long long s_scan[PLAN_THREADS];
+    int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
+    unsigned long long c = t < g.ntiles ? out.bin_cap[t] : 0;
+    s_scan[threadIdx.x] = c;
     __syncthreads();
     for (int off = 1; off < PLAN_THREADS; off <<= 1) {
         unsigned long long v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
@@ -518,14 +574,12 @@ __global__ void __launch_bounds__(PLAN_THREADS)
         s_scan[threadIdx.x] += v;
         __syncthreads();
     }
-    unsigned long long run = s_scan[threadIdx.x] - local;
-    for (int t = t0; t < t1; t++) {
-        out.bin_start[t] = (uint32_t)run;
-        run += out.bin_cap[t];
+    if (t < g.ntiles) {
+        out.bin_start[t] = (uint32_t)(T.blk[blockIdx.x] + s_scan[threadIdx.x] - c);
         out.bin_own[t] = 0;
         out.bin_in[t] = 0;
     }
-    if (threadIdx.x == 0) *out.stage_count = 0;
+    if (t == 0) *out.stage_count = 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1280,12 +1334,43 @@ static void free_buf(PartBuf &b) {
 
 static long default_stage_cap(long cap_bins) { return std::max<long>(65536, cap_bins / 16); }
 
-static int launch_plan_push(pic_species *s, uint32_t *err) {
+static PlanTmp plan_tmp(pic_species *s) {
+    const Grid &g = s->w->g;
+    size_t nt = (size_t)g.ntiles;
+    char *p = (char *)s->plan_mem;
+    PlanTmp T;
+    T.need = (uint32_t *)p;
+    T.slack = (uint32_t *)(p + 4 * nt);
+    T.alpha = (double *)(p + 8 * nt);
+    T.blk = (unsigned long long *)(p + 8 * nt + 16);
+    return T;
+}
+
+static size_t plan_bytes(const Grid &g) {
+    size_t nblk = ((size_t)g.ntiles + PLAN_THREADS - 1) / PLAN_THREADS;
+    return 8 * (size_t)g.ntiles + 16 + 16 * nblk;
+}
+
+// segment plan of out from the counts of in (mode 0) or from the histogram hist (mode 1)
+static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist, int mode, const PartBuf &out,
+                       uint32_t *err) {
     pic_world *w = s->w;
-    PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
-    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(w->g, view_of(in), nullptr, 0, view_of(out), err);
+    const Grid &g = w->g;
+    PlanTmp T = plan_tmp(s);
+    int nblk = (g.ntiles + PLAN_THREADS - 1) / PLAN_THREADS;
+    PartView vo = view_of(out);
+    k_plan_count<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, in, hist, mode, T);
+    k_plan_total<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T, nblk, (unsigned long long)out.cap_bins, err);
+    k_plan_caps<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, vo);
+    k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T, nblk);
+    k_plan_starts<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, vo);
     PIC_LAUNCH_CHECK();
     return PIC_OK;
+}
+
+static int launch_plan_push(pic_species *s, uint32_t *err) {
+    PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
+    return launch_plan(s, view_of(in), nullptr, 0, out, err);
 }
 
 // error flags of the last device operations live in DevScalars::counter slot of the world
@@ -1346,16 +1431,16 @@ static int sort_launch(pic_species *s) {
     PartView vin = view_of(in), vout = view_of(out);
     hipEvent_t ev;
     timer_begin(w, PIC_T_SORT, &ev);
-    // histogram into out.bin_in (zeroed), kept in out.bin_cap through the plan, which then
-    // sizes the own segments from it and zeroes the counters
+    // histogram into out.bin_in (zeroed); the plan sizes the own segments from it (its
+    // first kernel copies the counts) and its last kernel zeroes the counters
     PIC_HIP(hipMemsetAsync(out.bin_in, 0, sizeof(uint32_t) * g.ntiles, w->stream));
     unsigned grid = g.ntiles + STAGE_WGS * 4;
     k_sort_hist<PREC><<<grid, 256, 0, w->stream>>>(g, vin, out.bin_in);
     PIC_LAUNCH_CHECK();
-    PIC_HIP(hipMemcpyAsync(out.bin_cap, out.bin_in, sizeof(uint32_t) * g.ntiles, hipMemcpyDeviceToDevice,
-                           w->stream));
-    k_plan<<<1, PLAN_THREADS, 0, w->stream>>>(g, vin, out.bin_cap, 1, vout, err_flag(s));
-    PIC_LAUNCH_CHECK();
+    {
+        int rc = launch_plan(s, vin, out.bin_in, 1, out, err_flag(s));
+        if (rc) return rc;
+    }
     k_sort_scatter<PREC><<<grid, 256, 0, w->stream>>>(g, vin, vout, err_flag(s));
     PIC_LAUNCH_CHECK();
     timer_end(w, PIC_T_SORT, ev);
@@ -1434,6 +1519,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMalloc(&s->den_ave, sizeof(double) * g.nn));
     TRY(hipMalloc(&s->win, wbytes));
     TRY(hipMalloc(&s->win_used, g.ntiles));
+    TRY(hipMalloc(&s->plan_mem, plan_bytes(g)));
     TRY(hipMalloc(&s->diag_dev, sizeof(double) * 8));
     TRY(hipMalloc(&s->diag_part, sizeof(double) * NDIAG_SLOTS * (size_t)(g.ntiles + STAGE_WGS)));
     TRY(hipHostMalloc(&s->diag_host, sizeof(double) * 8));
@@ -1467,6 +1553,7 @@ int pic_species_destroy(pic_species *s) {
     hipFree(s->den_ave);
     hipFree(s->win);
     hipFree(s->win_used);
+    hipFree(s->plan_mem);
     hipFree(s->diag_dev);
     hipFree(s->diag_part);
     if (s->diag_host) hipHostFree(s->diag_host);
@@ -1661,6 +1748,9 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
         if (rc) return rc;
     }
     PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
+    // the push phase: segment plan of the output buffer, then the fused push
+    hipEvent_t ev;
+    timer_begin(w, PIC_T_PUSH, &ev);
     rc = launch_plan_push(s, err_flag(s));
     if (rc) return rc;
     PushParams P{};
@@ -1690,8 +1780,6 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     O.diag_part = s->diag_part;
     O.err = err_flag(s);
     unsigned grid = g.ntiles + STAGE_WGS;
-    hipEvent_t ev;
-    timer_begin(w, PIC_T_PUSH, &ev);
     if (s->prec == PIC_MIXED)
         k_push<PIC_MIXED><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
     else

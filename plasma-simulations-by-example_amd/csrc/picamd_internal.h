@@ -175,6 +175,7 @@ struct pic_species {
     int ave_samples = 0;
     void *win = nullptr;          // per-tile deposit windows (float or double)
     uint8_t *win_used = nullptr;
+    void *plan_mem = nullptr;     // scratch of the segment plan (per-tile counts, block sums)
     double *diag_dev = nullptr;   // [6] reduced diagnostics
     double *diag_part = nullptr;  // per-block partials of the last push (or diag pass)
     double *diag_host = nullptr;  // pinned
