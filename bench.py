@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--T-ion", type=float, default=1000.0)
+    ap.add_argument("--horizon", type=int, default=100,
+                    help="steps in all (untimed beyond warmup+steps) before the mixed-vs-FP64 "
+                         "field and history comparison")
     ap.add_argument("--push-only", action="store_true",
                     help="profiling mode: the same particle workload, pushes only (no solve), "
                          "plus a 1 GiB device copy that calibrates the HBM byte counters")
@@ -168,8 +171,10 @@ def initial_potential(P, w, solver_tol=1e-2):
     s.close()
 
 
-def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=False, phi=None):
-    """Builds the sphere case and times `steps` steps.  Returns a dict of results."""
+def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=False, phi=None, horizon=0):
+    """Builds the sphere case and times `steps` steps after `warmup` untimed ones, then runs
+    on untimed to `horizon` steps in all (when larger), recording the diagnostics history of
+    every step.  Returns a dict of results."""
     ni, nj, nk = grid
     w = P.World(ni, nj, nk)
     w.set_extents(Case.x0, Case.xm)
@@ -247,6 +252,19 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     out["particle_steps_per_s"] = np_sum / ((push_ms + dep_ms) * 1e-3)
     out["push_avg_ms"] = push_ms / max(push_calls, 1)
     out["phi_init"] = phi_init
+    # untimed continuation to the comparison horizon; diagnostics of every step
+    hist = []
+    for i in range(warmup + steps, max(horizon, warmup + steps)):
+        ok, its = step()
+        if not ok:
+            log(f"[bench]   step {i}: Newton did not converge")
+        d = ions.diag()  # getRealCount, getMomentum (3), getKE
+        hist.append([float(ions.np()), d[0], d[3], d[4], w.get_pe()])
+        if (i + 1) % 20 == 0:
+            log(f"[bench]   {'mixed' if precision == P.MIXED else 'fp64'} step {i + 1}: np {ions.np():.4g}, "
+                f"KE {d[4]:.6e}, PE {hist[-1][4]:.6e}")
+    out["history"] = np.array(hist)
+    out["steps_total"] = max(horizon, warmup + steps)
     if want_fields:
         out["phi"] = w.field(P.PHI)
         out["ef"] = w.field(P.EF)
@@ -309,8 +327,9 @@ def main():
     if args.push_only:
         push_only(P, args, grid)
         return
+    horizon = 0 if args.no_fp64 else args.horizon
     mixed = run_gpu(P, grid, args.particles, P.MIXED, args.steps, args.warmup, args.T_ion,
-                    want_fields=True)
+                    want_fields=True, horizon=horizon)
     log(f"[bench] mixed: {mixed['particle_steps_per_s']:.4e} particle-steps/s, "
         f"push {mixed['push_avg_ms']:.3f} ms/launch, {mixed['ms_per_step']:.1f} ms/step")
     fp64 = None
@@ -318,7 +337,7 @@ def main():
         # the same seeded particles and the same initial potential: the field differences
         # after warmup+steps steps are those of the particle arithmetic alone
         fp64 = run_gpu(P, grid, args.particles, P.FP64, args.steps, args.warmup, args.T_ion,
-                       want_fields=True, phi=mixed["phi_init"])
+                       want_fields=True, phi=mixed["phi_init"], horizon=horizon)
         log(f"[bench] fp64: {fp64['particle_steps_per_s']:.4e} particle-steps/s, "
             f"push {fp64['push_avg_ms']:.3f} ms/launch")
 
@@ -394,8 +413,18 @@ def main():
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
         line["mixed_vs_fp64_rel_l2"] = {
             "den": rel_l2(mixed["den"], fp64["den"]), "phi": rel_l2(mixed["phi"], fp64["phi"]),
-            "ef": rel_l2(mixed["ef"], fp64["ef"]), "after_steps": args.warmup + steps,
+            "ef": rel_l2(mixed["ef"], fp64["ef"]), "after_steps": mixed["steps_total"],
             "start": "same seeded particles, same initial phi"}
+        hm, hf = mixed["history"], fp64["history"]
+        if len(hm) and len(hm) == len(hf):
+            # largest relative difference over the history of each diagnostic
+            def hdiff(c, scale=None):
+                ref = np.abs(hf[:, c]) if scale is None else scale
+                return float(np.max(np.abs(hm[:, c] - hf[:, c]) / np.maximum(ref, 1e-300)))
+            line["mixed_vs_fp64_history"] = {
+                "steps": f"{args.warmup + steps + 1}-{mixed['steps_total']}",
+                "particle_count": hdiff(0), "real_count": hdiff(1), "kinetic_energy": hdiff(3),
+                "momentum_z": hdiff(2), "potential_energy": hdiff(4)}
     print(json.dumps(line), flush=True)
 
 

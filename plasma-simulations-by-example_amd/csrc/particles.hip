@@ -434,17 +434,23 @@ __device__ inline void deposit_lds(double *sW, int wi, int wj, int wk, T ox, T o
 // per-step bin plan: capacities of the output bins from the input counts
 // ---------------------------------------------------------------------------------------
 // Tile t gets an own segment of exactly n[t] slots (its input count: every particle that
-// stays fits) followed by an inflow segment of max(n/8, nb/4) + 32 slots, nb being the
-// largest face-neighbour population (one step of arrivals).  The inflow room is scaled down
-// uniformly when the buffer is short.  Mode 1 (counting sort) sizes bins from a histogram
-// with no inflow segment.  Five small launches over all tiles: counts, totals, capacities,
-// block offsets, starts (a single-workgroup plan was latency-bound at 0.3 ms for 65k tiles).
+// stays fits) followed by an inflow segment for one step of arrivals: the larger of
+// max(n/8, nb/4) (nb the largest face-neighbour population) and 9/8 of the arrivals the tile
+// saw in the last push (the raw reservation count, including arrivals that found the
+// segment full) plus the staged particles now in the tile, then + 32.  Sizing from observed
+// arrivals lets converging flows (the wake behind the sphere) get the room they need, and
+// counting staged particles drains the staging list instead of re-spilling it.  The inflow
+// room is scaled down uniformly when the buffer is short.  Mode 1 (counting sort) sizes bins
+// from a histogram with no inflow segment.  Five small launches over all tiles: counts,
+// totals, capacities, block offsets, starts (a single-workgroup plan was latency-bound at
+// 0.3 ms for 65k tiles).
 constexpr int PLAN_THREADS = 256;
 constexpr int PLAN_SCAN_THREADS = 1024;
 
 // scratch of the plan (per species): per-tile need and slack, per-block sums, alpha
 struct PlanTmp {
     uint32_t *need, *slack;
+    uint32_t *staged;         // staged particles per current tile (zeroed again by k_plan_caps)
     unsigned long long *blk;  // [2 * nblocks]
     double *alpha;            // [0] alpha, [1] 1 when the buffer is short
 };
@@ -454,8 +460,8 @@ __device__ inline uint32_t tile_count(const PartView &v, int t) {
     return b.c0 + b.c1;
 }
 
-__device__ inline void plan_sizes(const Grid &g, const PartView &in, const uint32_t *hist, int mode, int t,
-                                  uint32_t &n, uint32_t &s) {
+__device__ inline void plan_sizes(const Grid &g, const PartView &in, const uint32_t *hist, const uint32_t *staged,
+                                  int mode, int t, uint32_t &n, uint32_t &s) {
     if (mode == 0) {
         n = tile_count(in, t);
         uint32_t nb = 0;
@@ -466,7 +472,8 @@ __device__ inline void plan_sizes(const Grid &g, const PartView &in, const uint3
         if (ty < g.nty - 1) nb = max(nb, tile_count(in, t + g.ntx));
         if (tz > 0) nb = max(nb, tile_count(in, t - g.ntx * g.nty));
         if (tz < g.ntz - 1) nb = max(nb, tile_count(in, t + g.ntx * g.nty));
-        s = max(n / 8, nb / 4) + 32;
+        uint32_t arrivals = in.bin_in[t] + staged[t];
+        s = max(max(n / 8, nb / 4), arrivals + arrivals / 8) + 32;
     } else {
         n = hist[t];
         s = 0;
@@ -485,13 +492,25 @@ __device__ inline unsigned long long block_sum_u64(unsigned long long v, unsigne
     return r;
 }
 
+// staged particles per current tile (the staging list of the input buffer)
+template <int PREC>
+__global__ void __launch_bounds__(256) k_plan_staged(Grid g, PartView in, uint32_t *staged) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
+    for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < ns; q += gridDim.x * 256) {
+        PT p;
+        load_p(in, in.cap_bins + q, p);
+        atomicAdd(&staged[tile_of(g, p)], 1u);
+    }
+}
+
 __global__ void __launch_bounds__(PLAN_THREADS)
     k_plan_count(Grid g, PartView in, const uint32_t *hist, int mode, PlanTmp T) {
     __shared__ unsigned long long sh[PLAN_THREADS / 64];
     int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
     uint32_t n = 0, sl = 0;
     if (t < g.ntiles) {
-        plan_sizes(g, in, hist, mode, t, n, sl);
+        plan_sizes(g, in, hist, T.staged, mode, t, n, sl);
         T.need[t] = n;
         T.slack[t] = sl;
     }
@@ -534,6 +553,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_caps(Grid g, PlanTmp T, P
         c = own + (uint32_t)(T.alpha[0] * (double)T.slack[t]);
         out.bin_cap[t] = c;
         out.bin_off[t] = own;
+        T.staged[t] = 0;  // ready for the next plan
     }
     unsigned long long a = block_sum_u64(c, sh);
     if (threadIdx.x == 0) T.blk[blockIdx.x] = a;
@@ -1341,14 +1361,15 @@ static PlanTmp plan_tmp(pic_species *s) {
     PlanTmp T;
     T.need = (uint32_t *)p;
     T.slack = (uint32_t *)(p + 4 * nt);
-    T.alpha = (double *)(p + 8 * nt);
-    T.blk = (unsigned long long *)(p + 8 * nt + 16);
+    T.staged = (uint32_t *)(p + 8 * nt);
+    T.alpha = (double *)(p + ((12 * nt + 7) & ~(size_t)7));
+    T.blk = (unsigned long long *)((char *)T.alpha + 16);
     return T;
 }
 
 static size_t plan_bytes(const Grid &g) {
     size_t nblk = ((size_t)g.ntiles + PLAN_THREADS - 1) / PLAN_THREADS;
-    return 8 * (size_t)g.ntiles + 16 + 16 * nblk;
+    return ((12 * (size_t)g.ntiles + 7) & ~(size_t)7) + 16 + 16 * nblk;
 }
 
 // segment plan of out from the counts of in (mode 0) or from the histogram hist (mode 1)
@@ -1359,6 +1380,12 @@ static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist,
     PlanTmp T = plan_tmp(s);
     int nblk = (g.ntiles + PLAN_THREADS - 1) / PLAN_THREADS;
     PartView vo = view_of(out);
+    if (mode == 0) {
+        if (s->prec == PIC_MIXED)
+            k_plan_staged<PIC_MIXED><<<STAGE_WGS, 256, 0, w->stream>>>(g, in, T.staged);
+        else
+            k_plan_staged<PIC_FP64><<<STAGE_WGS, 256, 0, w->stream>>>(g, in, T.staged);
+    }
     k_plan_count<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, in, hist, mode, T);
     k_plan_total<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T, nblk, (unsigned long long)out.cap_bins, err);
     k_plan_caps<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, vo);
@@ -1528,6 +1555,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMemsetAsync(s->den_extra, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->den_ave, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->win_used, 0, g.ntiles, w->stream));
+    TRY(hipMemsetAsync(s->plan_mem, 0, plan_bytes(g), w->stream));  // staged counts start at 0
     TRY(hipMemsetAsync(s->diag_dev, 0, sizeof(double) * 8, w->stream));
     TRY(hipStreamSynchronize(w->stream));
 #undef TRY

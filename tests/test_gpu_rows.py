@@ -163,3 +163,34 @@ def test_capacity_grows_with_appends():
     sp.compute_number_density()
     assert sp.np() == total
     assert abs(np.sum(sp.den() * g.node_volumes()) / total - 1) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [P.MIXED, P.FP64])
+def test_converging_flow_keeps_every_particle(prec):
+    """Particles streaming toward the centre of a reflecting box pile into the central tiles
+    far beyond one neighbour's share.  The bin plan sizes inflow segments from the arrivals it
+    observed and from the staged particles, so the overflow list drains instead of growing
+    step after step; no particle is dropped (a dropped one raises a capacity error)."""
+    n_nodes = (65, 65, 65)
+    x0, xm = (-0.1, -0.1, -0.1), (0.1, 0.1, 0.1)
+    w = P.World(*n_nodes)
+    w.set_extents(x0, xm)
+    w.set_field(P.EF, np.zeros(3 * 65 ** 3))
+    dh = 0.2 / 64
+    dt = 1e-7
+    n = 4_000_000
+    rng = np.random.default_rng(21)
+    pos = -0.1 + 0.2 * rng.random((n, 3))
+    r = np.linalg.norm(pos, axis=1, keepdims=True)
+    vel = -pos / np.maximum(r, 1e-6) * (1.5 * dh / dt)  # 1.5 cells per step toward the centre
+    mpw = np.full(n, 1e4)
+    sp = P.Species(w, 16 * O.AMU, O.QE, 1e4, prec, capacity=n)
+    sp.set_particles(pos, vel, mpw)
+    sp.sort()
+    for _ in range(40):
+        sp.advance(P.REFLECT, dt)
+        sp.compute_number_density()
+    assert sp.np() == n
+    assert abs(sp.diag()[0] / (n * 1e4) - 1) < 1e-6
+    g = O.OGrid(*n_nodes, x0, xm)
+    assert abs(np.sum(sp.den() * g.node_volumes()) / (n * 1e4) - 1) < 1e-6
