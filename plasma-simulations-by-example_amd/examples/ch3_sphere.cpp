@@ -1,7 +1,7 @@
 // Flow past a sphere with Boltzmann electrons: cold (or warm) O+ beam injected at z=0,
 // absorbing sphere and walls.  The call sequence is that of ch3/ver2/Main.cpp:16-91,
 // written against the drop-in classes.
-//   usage: ch3_sphere [num_ts=400] [solver=PCG|GS|QN] [phi_sphere=-100] [T_ion=0] [seed=1]
+//   usage: ch3_sphere [num_ts=400] [solver=PCG|GS|QN|PCGS|MG] [phi_sphere=-100] [T_ion=0] [seed=1]
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -22,6 +22,8 @@ int main(int argc, char *argv[]) {
     SolverType st = PCG;
     if (argc > 2 && !strcmp(argv[2], "GS")) st = GS;
     if (argc > 2 && !strcmp(argv[2], "QN")) st = QN;
+    if (argc > 2 && !strcmp(argv[2], "PCGS")) st = PCGS;
+    if (argc > 2 && !strcmp(argv[2], "MG")) st = MG;
     double phi_sphere = argc > 3 ? atof(argv[3]) : -100;
     double T_ion = argc > 4 ? atof(argv[4]) : 0.0;
     rnd.seed(argc > 5 ? (unsigned)atoi(argv[5]) : 1u);

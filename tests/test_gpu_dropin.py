@@ -89,3 +89,15 @@ def test_ch3_sphere_dropin_runs_and_conserves(tmp_path):
     for r in rows:
         assert abs(float(r["real_count.O+"]) - 50.0 * float(r["mp_count.O+"])) < 1e-6 * float(
             r["real_count.O+"])
+
+
+def test_ch3_sphere_dropin_mg_matches_pcg(tmp_path):
+    """The same ch3 main with SolverType MG (multigrid-preconditioned CG on the reduced
+    system) follows the PCG run: both stop on the reference's tolerances, so the potentials
+    differ by at most the solver tolerance and the diagnostics histories agree closely."""
+    _, rows_p = run_example(tmp_path / "pcg", "ch3_sphere", 30, "PCG", -100, 0, 3)
+    _, rows_m = run_example(tmp_path / "mg", "ch3_sphere", 30, "MG", -100, 0, 3)
+    assert len(rows_p) == len(rows_m)
+    for rp, rm in zip(rows_p, rows_m):
+        assert abs(float(rm["mp_count.O+"]) - float(rp["mp_count.O+"])) <= 2e-3 * float(rp["mp_count.O+"])
+        assert abs(float(rm["PE"]) - float(rp["PE"])) <= 2e-3 * abs(float(rp["PE"]))
