@@ -40,7 +40,7 @@ struct MgLevel {
     PcgParams Q;             // coefficients at this spacing
     uint8_t *oid = nullptr;  // object ids (level 0: the world's)
     uint8_t *mask = nullptr; // bits 0-5 regular neighbours, bit 6 regular (level 0: the solver's)
-    float *diag = nullptr;   // reduced diagonal
+    float *idiag = nullptr;  // reciprocal of the reduced diagonal
     float *boltz = nullptr;  // Boltzmann derivative P (levels >= 1)
     float *x = nullptr, *r = nullptr;
     float *b = nullptr;      // right-hand side (levels >= 1; level 0 reads the CG residual)
@@ -86,15 +86,15 @@ __global__ void __launch_bounds__(256)
     P_c[u] = (float)P_f[f];
 }
 
-// level-0 diagonal in FP32
+// level-0 reciprocal diagonal in FP32
 __global__ void __launch_bounds__(256) k_mg_tof(long n, const double *a, float *b) {
     long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u < n) b[u] = (float)a[u];
+    if (u < n) b[u] = (float)(1.0 / a[u]);
 }
 
 // mask and reduced diagonal of a coarse level (the operator part of k_nrs_F)
 __global__ void __launch_bounds__(256)
-    k_mg_ops(PcgParams Q, const uint8_t *oid, const float *P, uint8_t *mask, float *diag) {
+    k_mg_ops(PcgParams Q, const uint8_t *oid, const float *P, uint8_t *mask, float *idiag) {
     long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256)
     node_ijk(g, u, i, j, k);
     if (node_type(g, oid, u, i, j, k) != NT_REG) {
         mask[u] = 0;
-        diag[u] = 1;
+        idiag[u] = 1;
         return;
     }
     double dg = -2.0 * (Q.idx2 + Q.idy2 + Q.idz2) - (double)P[u];
@@ -115,20 +115,10 @@ __global__ void __launch_bounds__(256)
         else if (tv == NT_NEU) dg += nb_coef(Q, d);
     }
     mask[u] = m;
-    diag[u] = (float)dg;
+    idiag[u] = (float)(1.0 / dg);
 }
 
 // ---- per-node steps (shared by the per-level kernels and the fused coarse cycle) ----------
-// first half-sweep from x = 0: red regular nodes x = b/diag, every other node 0
-template <typename TB>
-__device__ inline void mg_smooth_first_node(const Grid &g, const uint8_t *mask, const float *diag, float *x,
-                                            const TB *b, long u) {
-    int i, j, k;
-    node_ijk(g, u, i, j, k);
-    bool red = ((i + j + k) & 1) == 0;
-    x[u] = (red && (mask[u] & 64)) ? (float)b[u] / diag[u] : 0.0f;
-}
-
 // Iterates x, r are zero off the regular nodes (smoothers and prolongation write regular
 // nodes only, and the first half-sweep of a V-cycle zeroes the rest), and every regular node
 // is interior.  So the six neighbours of an interior node are loaded unconditionally and
@@ -145,9 +135,35 @@ __device__ inline float nb_sum(const Grid &g, const Coef6 &C, const float *x, lo
            C.c[4] * x[u + sj] + C.c[5] * x[u + sk];
 }
 
+// The first sweep of a level from x = 0, red then black, in one pass: a red regular node
+// takes b/diag; a black regular node the Gauss-Seidel update from those red values, computed
+// from b/diag at its neighbours (b, like x, is zero off the regular nodes); other nodes 0.
+template <typename TB>
+__device__ inline void mg_pre_node(const Grid &g, const Coef6 &C, const uint8_t *mask, const float *idiag, float *x,
+                                   const TB *b, long u) {
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
+    float v = 0.0f;
+    if (interior(g, i, j, k)) {
+        uint8_t m = mask[u];
+        float bu = (float)b[u], du = idiag[u];
+        if (((i + j + k) & 1) == 0) {
+            v = bu * du;
+        } else {
+            long sj = g.ni, sk = (long)g.ni * g.nj;
+            float s = C.c[0] * ((float)b[u - sk] * idiag[u - sk]) + C.c[1] * ((float)b[u - sj] * idiag[u - sj]) +
+                      C.c[2] * ((float)b[u - 1] * idiag[u - 1]) + C.c[3] * ((float)b[u + 1] * idiag[u + 1]) +
+                      C.c[4] * ((float)b[u + sj] * idiag[u + sj]) + C.c[5] * ((float)b[u + sk] * idiag[u + sk]);
+            v = (bu - s) * du;
+        }
+        if (!(m & 64)) v = 0.0f;
+    }
+    x[u] = v;
+}
+
 // Gauss-Seidel update of the t-th node of one colour (x-fastest, every second node of a row)
 template <typename TB>
-__device__ inline void mg_smooth_node(const Grid &g, const Coef6 &C, const uint8_t *mask, const float *diag,
+__device__ inline void mg_smooth_node(const Grid &g, const Coef6 &C, const uint8_t *mask, const float *idiag,
                                       float *x, const TB *b, int color, long t) {
     unsigned half = (unsigned)(g.ni + 1) / 2;
     unsigned row = (unsigned)t / half, ih = (unsigned)t - row * half;
@@ -158,22 +174,22 @@ __device__ inline void mg_smooth_node(const Grid &g, const Coef6 &C, const uint8
     long u = i + (long)g.ni * row;
     uint8_t m = mask[u];
     float s = nb_sum(g, C, x, u);
-    float bb = (float)b[u], dg = diag[u];
-    if (m & 64) x[u] = (bb - s) / dg;
+    float bb = (float)b[u], du = idiag[u];
+    if (m & 64) x[u] = (bb - s) * du;
 }
 
 __device__ inline long smooth_count(const Grid &g) { return (long)((g.ni + 1) / 2) * g.nj * g.nk; }
 
 // r = b - J x on regular nodes, 0 elsewhere
 template <typename TB>
-__device__ inline void mg_residual_node(const Grid &g, const Coef6 &C, const uint8_t *mask, const float *diag,
+__device__ inline void mg_residual_node(const Grid &g, const Coef6 &C, const uint8_t *mask, const float *idiag,
                                         const float *x, const TB *b, float *r, long u) {
     int i, j, k;
     node_ijk(g, u, i, j, k);
     float v = 0.0f;
     if (interior(g, i, j, k)) {
         uint8_t m = mask[u];
-        float s = diag[u] * x[u] + nb_sum(g, C, x, u);
+        float s = x[u] / idiag[u] + nb_sum(g, C, x, u);
         float bb = (float)b[u];
         if (m & 64) v = bb - s;
     }
@@ -260,28 +276,28 @@ __device__ inline void mg_prolong_node(const Grid &gf, const Grid &gc, const uin
 // ---- per-level kernels (one node per thread, XCD-remapped workgroups) ---------------------
 template <typename TB>
 __global__ void __launch_bounds__(256)
-    k_mg_smooth_first(Grid g, const uint8_t *mask, const float *diag, float *x, const TB *b, const DevScalars *sc) {
+    k_mg_pre(Grid g, Coef6 C, const uint8_t *mask, const float *idiag, float *x, const TB *b, const DevScalars *sc) {
     if (sc->done) return;
     long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u < g.nn) mg_smooth_first_node(g, mask, diag, x, b, u);
+    if (u < g.nn) mg_pre_node(g, C, mask, idiag, x, b, u);
 }
 
 template <typename TB>
 __global__ void __launch_bounds__(256)
-    k_mg_smooth(Grid g, Coef6 C, const uint8_t *mask, const float *diag, float *x, const TB *b, int color,
+    k_mg_smooth(Grid g, Coef6 C, const uint8_t *mask, const float *idiag, float *x, const TB *b, int color,
                 const DevScalars *sc) {
     if (sc->done) return;
     long t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t < smooth_count(g)) mg_smooth_node(g, C, mask, diag, x, b, color, t);
+    if (t < smooth_count(g)) mg_smooth_node(g, C, mask, idiag, x, b, color, t);
 }
 
 template <typename TB>
 __global__ void __launch_bounds__(256)
-    k_mg_residual(Grid g, Coef6 C, const uint8_t *mask, const float *diag, const float *x, const TB *b, float *r,
+    k_mg_residual(Grid g, Coef6 C, const uint8_t *mask, const float *idiag, const float *x, const TB *b, float *r,
                   const DevScalars *sc) {
     if (sc->done) return;
     long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u < g.nn) mg_residual_node(g, C, mask, diag, x, b, r, u);
+    if (u < g.nn) mg_residual_node(g, C, mask, idiag, x, b, r, u);
 }
 
 __global__ void __launch_bounds__(256)
@@ -320,7 +336,7 @@ struct MgDev {
     Grid g;
     Coef6 C;
     const uint8_t *oid, *mask;
-    const float *diag;
+    const float *idiag;
     float *x, *r, *b;
 };
 
@@ -333,12 +349,10 @@ __global__ void __launch_bounds__(MG_FUSE_THREADS)
     const long T = MG_FUSE_THREADS;
     for (int l = l0; l < nl - 1; l++) {
         const MgDev &f = L[l], &c = L[l + 1];
-        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_smooth_first_node(f.g, f.mask, f.diag, f.x, f.b, u);
+        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_pre_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, u);
         __syncthreads();
-        for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
-            mg_smooth_node(f.g, f.C, f.mask, f.diag, f.x, f.b, 1, t);
-        __syncthreads();
-        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_residual_node(f.g, f.C, f.mask, f.diag, f.x, f.b, f.r, u);
+        for (long u = threadIdx.x; u < f.g.nn; u += T)
+            mg_residual_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, f.r, u);
         __syncthreads();
         for (long u = threadIdx.x; u < c.g.nn; u += T) mg_restrict_node(f.g, c.g, f.r, c.r, u);
         __syncthreads();
@@ -349,12 +363,11 @@ __global__ void __launch_bounds__(MG_FUSE_THREADS)
         const MgDev &c = L[nl - 1];
         auto half_sweep = [&](int color) {
             for (long t = threadIdx.x; t < smooth_count(c.g); t += T)
-                mg_smooth_node(c.g, c.C, c.mask, c.diag, c.x, c.b, color, t);
+                mg_smooth_node(c.g, c.C, c.mask, c.idiag, c.x, c.b, color, t);
             __syncthreads();
         };
-        for (long u = threadIdx.x; u < c.g.nn; u += T) mg_smooth_first_node(c.g, c.mask, c.diag, c.x, c.b, u);
+        for (long u = threadIdx.x; u < c.g.nn; u += T) mg_pre_node(c.g, c.C, c.mask, c.idiag, c.x, c.b, u);
         __syncthreads();
-        half_sweep(1);
         // (red, black) pairs, then as many (black, red) pairs: a symmetric coarsest solve
         for (int it = 1; it < sweeps; it++) {
             half_sweep(0);
@@ -372,10 +385,10 @@ __global__ void __launch_bounds__(MG_FUSE_THREADS)
         for (long u = threadIdx.x; u < f.g.nn; u += T) mg_prolong_node(f.g, c.g, f.mask, c.x, f.x, u);
         __syncthreads();
         for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
-            mg_smooth_node(f.g, f.C, f.mask, f.diag, f.x, f.b, 1, t);
+            mg_smooth_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, 1, t);
         __syncthreads();
         for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
-            mg_smooth_node(f.g, f.C, f.mask, f.diag, f.x, f.b, 0, t);
+            mg_smooth_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, 0, t);
         __syncthreads();
     }
 }
@@ -402,7 +415,7 @@ void mg_destroy(MgHierarchy *H) {
     if (!H) return;
     hipFree(H->dev);
     for (auto &l : H->L) {
-        hipFree(l.diag);
+        hipFree(l.idiag);
         hipFree(l.x);
         hipFree(l.r);
         if (!l.owned) continue;
@@ -442,7 +455,7 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
     for (size_t l = 0; l < H->L.size() && e == hipSuccess; l++) {
         MgLevel &c = H->L[l];
         size_t n = (size_t)c.g.nn;
-        if (e == hipSuccess) e = hipMalloc(&c.diag, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&c.idiag, n * sizeof(float));
         if (e == hipSuccess) e = hipMalloc(&c.x, n * sizeof(float));
         if (e == hipSuccess) e = hipMalloc(&c.r, n * sizeof(float));
         if (!c.owned) continue;
@@ -479,7 +492,7 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
             }
             d.oid = v.oid;
             d.mask = v.mask;
-            d.diag = v.diag;
+            d.idiag = v.idiag;
             d.x = v.x;
             d.r = v.r;
             d.b = v.b;
@@ -502,7 +515,7 @@ int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const doubl
         MgLevel &c = H->L[l];
         c.Q = level_params(Q0, c.g);
         if (l == 0) {
-            k_mg_tof<<<nbk8(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.diag);
+            k_mg_tof<<<nbk8(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.idiag);
             continue;
         }
         const MgLevel &f = H->L[l - 1];
@@ -510,7 +523,7 @@ int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const doubl
             k_mg_inject<double><<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, boltz0, c.oid, c.boltz);
         else
             k_mg_inject<float><<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, f.boltz, c.oid, c.boltz);
-        k_mg_ops<<<nbk8(c.g.nn), 256, 0, st>>>(c.Q, c.oid, c.boltz, c.mask, c.diag);
+        k_mg_ops<<<nbk8(c.g.nn), 256, 0, st>>>(c.Q, c.oid, c.boltz, c.mask, c.idiag);
     }
     PIC_LAUNCH_CHECK();
     return PIC_OK;
@@ -521,9 +534,8 @@ template <typename TB>
 static void down(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScalars *sc) {
     MgLevel &f = H->L[l], &c = H->L[l + 1];
     Coef6 C = coef6(f.Q);
-    k_mg_smooth_first<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b, sc);
-    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 1, sc);
-    k_mg_residual<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, f.r, sc);
+    k_mg_pre<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, sc);
+    k_mg_residual<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, f.r, sc);
     k_mg_restrict<<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.r, c.r, sc);
     k_mg_fold<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
 }
@@ -535,8 +547,8 @@ static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScal
     Coef6 C = coef6(f.Q);
     k_mg_extend<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
     k_mg_prolong<<<nbk8(f.g.nn), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
-    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 1, sc);
-    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b, 0, sc);
+    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
+    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
 }
 
 // x0 = V(b0): one V-cycle from a zero initial guess on every level; x0 is level 0's
@@ -547,10 +559,9 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, cons
     if (nl == 1) {  // no coarsening possible: the preconditioner is one symmetric sweep pair
         MgLevel &f = H->L[0];
         Coef6 C = coef6(f.Q);
-        k_mg_smooth_first<double><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.diag, f.x, b0, sc);
-        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 1, sc);
-        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 1, sc);
-        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.diag, f.x, b0, 0, sc);
+        k_mg_pre<double><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, sc);
+        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, 1, sc);
+        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, 0, sc);
         PIC_LAUNCH_CHECK();
         return PIC_OK;
     }
@@ -568,16 +579,15 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, cons
     {
         MgLevel &c = H->L[nl - 1];
         Coef6 C = coef6(c.Q);
-        k_mg_smooth_first<float><<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.mask, c.diag, c.x, c.b, sc);
-        k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 1, sc);
+        k_mg_pre<float><<<nbk8(c.g.nn), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, sc);
         // symmetric sweeps: (red, black) pairs, then as many (black, red) pairs
         for (int it = 1; it < H->coarse_sweeps; it++) {
-            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 0, sc);
-            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 1, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, 0, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, 1, sc);
         }
         for (int it = 0; it < H->coarse_sweeps; it++) {
-            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 1, sc);
-            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.diag, c.x, c.b, 0, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, 1, sc);
+            k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, 0, sc);
         }
     }
     for (int l = nl - 2; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);
