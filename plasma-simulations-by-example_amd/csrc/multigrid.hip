@@ -255,12 +255,20 @@ __device__ inline void mg_extend_node(const Grid &g, const uint8_t *oid, float *
     x[u] = x[chain_target(g, oid, u, i, j, k)];
 }
 
-// x_f += trilinear interpolation of x_c, on regular fine nodes
+// x_f += trilinear interpolation of x_c at the t-th red node of a row (colour-strided like
+// mg_smooth_node).  Only red nodes: the post-smoothing black half-sweep that follows
+// overwrites every black value without reading it, and the red half-sweep after it reads
+// only black values, so a correction written to black nodes would never be read.
 __device__ inline void mg_prolong_node(const Grid &gf, const Grid &gc, const uint8_t *mask_f, const float *xc,
-                                       float *xf, long u) {
+                                       float *xf, long t) {
+    unsigned half = (unsigned)(gf.ni + 1) / 2;
+    unsigned row = (unsigned)t / half, ih = (unsigned)t - row * half;
+    unsigned kk = row / (unsigned)gf.nj;
+    int j = (int)(row - kk * (unsigned)gf.nj), k = (int)kk;
+    int i = (int)(2 * ih) + ((j + k) & 1);
+    if (i >= gf.ni) return;
+    long u = i + (long)gf.ni * row;
     if (!(mask_f[u] & 64)) return;
-    int i, j, k;
-    node_ijk(gf, u, i, j, k);
     int I = i >> 1, J = j >> 1, K = k >> 1;
     int ai = i & 1, aj = j & 1, ak = k & 1;
     long sj = gc.ni, sk = (long)gc.ni * gc.nj;
@@ -289,6 +297,45 @@ __global__ void __launch_bounds__(256)
     if (sc->done) return;
     long t = xcd_block() * blockDim.x + threadIdx.x;
     if (t < smooth_count(g)) mg_smooth_node(g, C, mask, idiag, x, b, color, t);
+}
+
+// The last half-sweep of a V-cycle (red, finest level) fused with the CG product g.s:
+// thread t updates the t-th red node of its row and adds g*s for that node and for the
+// black node beside it (final since the previous half-sweep), so every node is counted
+// once.  One partial per workgroup.
+__global__ void __launch_bounds__(256)
+    k_mg_smooth_dot(Grid g, Coef6 C, const uint8_t *mask, const float *idiag, float *x, const double *b,
+                    double *partials, const DevScalars *sc) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    long t = xcd_block() * blockDim.x + threadIdx.x;
+    double acc = 0;
+    if (t < smooth_count(g)) {
+        unsigned half = (unsigned)(g.ni + 1) / 2;
+        unsigned row = (unsigned)t / half, ih = (unsigned)t - row * half;
+        unsigned kk = row / (unsigned)g.nj;
+        int j = (int)(row - kk * (unsigned)g.nj), k = (int)kk;
+        int par = (j + k) & 1;
+        int i = (int)(2 * ih) + par, ib = (int)(2 * ih) + 1 - par;
+        if (i < g.ni && interior(g, i, j, k)) {
+            long u = i + (long)g.ni * row;
+            uint8_t m = mask[u];
+            float s = nb_sum(g, C, x, u);
+            double bb = b[u];
+            float du = idiag[u];
+            if (m & 64) {
+                float xn = ((float)bb - s) * du;
+                x[u] = xn;
+                acc += bb * (double)xn;
+            }
+        }
+        if (ib < g.ni) {
+            long ub = ib + (long)g.ni * row;
+            acc += b[ub] * (double)x[ub];
+        }
+    }
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
 template <typename TB>
@@ -323,8 +370,8 @@ __global__ void __launch_bounds__(256) k_mg_extend(Grid g, const uint8_t *oid, f
 __global__ void __launch_bounds__(256)
     k_mg_prolong(Grid gf, Grid gc, const uint8_t *mask_f, const float *xc, float *xf, const DevScalars *sc) {
     if (sc->done) return;
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u < gf.nn) mg_prolong_node(gf, gc, mask_f, xc, xf, u);
+    long t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t < smooth_count(gf)) mg_prolong_node(gf, gc, mask_f, xc, xf, t);
 }
 
 // ---- the coarse levels in one workgroup -------------------------------------------------
@@ -382,7 +429,7 @@ __global__ void __launch_bounds__(MG_FUSE_THREADS)
         const MgDev &f = L[l], &c = L[l + 1];
         for (long u = threadIdx.x; u < c.g.nn; u += T) mg_extend_node(c.g, c.oid, c.x, u);
         __syncthreads();
-        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_prolong_node(f.g, c.g, f.mask, c.x, f.x, u);
+        for (long t = threadIdx.x; t < smooth_count(f.g); t += T) mg_prolong_node(f.g, c.g, f.mask, c.x, f.x, t);
         __syncthreads();
         for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
             mg_smooth_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, 1, t);
@@ -540,28 +587,40 @@ static void down(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevSc
     k_mg_fold<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
 }
 
-// prolongation of level l+1 into level l, then post-smoothing (black, red)
+// the last (red) half-sweep of the finest level, with the product g.s when asked for
+static void last_sweep(MgLevel &f, const double *b0, double *dot_partials, hipStream_t st, const DevScalars *sc) {
+    Coef6 C = coef6(f.Q);
+    if (dot_partials)
+        k_mg_smooth_dot<<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, dot_partials, sc);
+    else
+        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, 0, sc);
+}
+
+// prolongation of level l+1 into level l, then post-smoothing (black, red); on level 0 the
+// red half-sweep is last_sweep
 template <typename TB>
 static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScalars *sc) {
     MgLevel &f = H->L[l], &c = H->L[l + 1];
     Coef6 C = coef6(f.Q);
     k_mg_extend<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
-    k_mg_prolong<<<nbk8(f.g.nn), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
+    k_mg_prolong<<<smooth_blocks(f.g), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
     k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
-    k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
+    if (l > 0) k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
 }
 
 // x0 = V(b0): one V-cycle from a zero initial guess on every level; x0 is level 0's
-// solution (FP32), b0 the CG residual (FP64)
-int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, const DevScalars *sc) {
+// solution (FP32), b0 the CG residual (FP64).  With dot_partials, the last half-sweep also
+// writes per-workgroup partials of b0.x0 (mg_dot_partials of them).
+int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, double *dot_partials, hipStream_t st,
+              const DevScalars *sc) {
     const int nl = (int)H->L.size();
     *x0 = H->L[0].x;
+    MgLevel &f0 = H->L[0];
     if (nl == 1) {  // no coarsening possible: the preconditioner is one symmetric sweep pair
-        MgLevel &f = H->L[0];
-        Coef6 C = coef6(f.Q);
-        k_mg_pre<double><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, sc);
-        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, 1, sc);
-        k_mg_smooth<double><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b0, 0, sc);
+        Coef6 C = coef6(f0.Q);
+        k_mg_pre<double><<<nbk8(f0.g.nn), 256, 0, st>>>(f0.g, C, f0.mask, f0.idiag, f0.x, b0, sc);
+        k_mg_smooth<double><<<smooth_blocks(f0.g), 256, 0, st>>>(f0.g, C, f0.mask, f0.idiag, f0.x, b0, 1, sc);
+        last_sweep(f0, b0, dot_partials, st, sc);
         PIC_LAUNCH_CHECK();
         return PIC_OK;
     }
@@ -572,6 +631,7 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, cons
         k_mg_coarse<<<1, MG_FUSE_THREADS, 0, st>>>(H->dev, l0, nl, H->coarse_sweeps, sc);
         for (int l = l0 - 1; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);
         up<double>(H, 0, b0, st, sc);
+        last_sweep(f0, b0, dot_partials, st, sc);
         PIC_LAUNCH_CHECK();
         return PIC_OK;
     }
@@ -592,9 +652,13 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, cons
     }
     for (int l = nl - 2; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);
     up<double>(H, 0, b0, st, sc);
+    last_sweep(f0, b0, dot_partials, st, sc);
     PIC_LAUNCH_CHECK();
     return PIC_OK;
 }
+
+// number of partials the fused product of mg_vcycle writes
+int mg_dot_partials(const MgHierarchy *H) { return (int)smooth_blocks(H->L[0].g); }
 
 int mg_levels(const MgHierarchy *H) { return H ? (int)H->L.size() : 0; }
 

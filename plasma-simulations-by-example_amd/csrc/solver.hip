@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(256)
 //   init:    g = J y - F;  s = V(g);  a = g.s;  d_0 = 0
 //   pass A:  d = mu d_old - s (at the 7 stencil points), y += lam d_old, z = J d, partials d.z
 //   pass B:  g += (a/b) z, partials g.g (convergence: RMS(g) < tol)
-//   s = V(g), partials g.s:  mu = a'/a
+//   s = V(g), with the partials of g.s from its last sweep:  mu = a'/a
 // Scalars: red[0] a, red[1] b, red[4] RMS(g), red[6] mu, red[7] lam = a/b.
 __global__ void __launch_bounds__(256)
     k_mgpcg_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
@@ -673,17 +673,6 @@ __global__ void __launch_bounds__(256)
     uint8_t m = mask[u];
     gv[u] = (m & 64) ? jr_apply(Q, u, m, jr[u], y) - F[u] : 0.0;
     d0[u] = 0;
-}
-
-__global__ void __launch_bounds__(256)
-    k_mgpcg_dot(long nn, const double *a, const float *b, double *partials, const DevScalars *sc) {
-    __shared__ double sh[4];
-    if (sc->done) return;
-    NodeChunk ch = pcg_chunk(nn);
-    double s = 0;
-    for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) s += a[u] * (double)b[u];
-    double r = block_sum(s, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
 __global__ void __launch_bounds__(256)
@@ -1132,7 +1121,9 @@ static int gsls_linear(pic_solver *s, const PcgParams &Q) {
 // Newton iteration of solveNRPCG (:225-296) on the Neumann-reduced system
 int mg_build(pic_solver *s, MgHierarchy **out);
 int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, hipStream_t st);
-int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, hipStream_t st, const DevScalars *sc);
+int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, double *dot_partials, hipStream_t st,
+              const DevScalars *sc);
+int mg_dot_partials(const MgHierarchy *H);
 void mg_destroy(MgHierarchy *H);
 
 // solvePCGLinear on the reduced system with one V-cycle as the preconditioner
@@ -1150,11 +1141,13 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     hipStream_t st = w->stream;
     k_reset_flags<<<1, 1, 0, st>>>(w->scal);
     k_mgpcg_init<<<nb8, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0]);
-    float *sv = nullptr;  // s = V(g), FP32
-    rc = mg_vcycle(s->mg, s->g, &sv, st, w->scal);
+    // s = V(g) in FP32; the V-cycle's last sweep also writes the partials of g.s
+    float *sv = nullptr;
+    const long ndot = mg_dot_partials(s->mg);
+    rc = mg_vcycle(s->mg, s->g, &sv, w->partials, st, w->scal);
     if (rc) return rc;
-    k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, sv, w->partials, w->scal);
-    k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 0, nu, Q.tol, 0, w->scal);
+    k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, ndot, folded, w->scal);
+    k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 0, nu, Q.tol, 0, w->scal);
     // a few iterations are queued between host checks of the convergence flag; kernels of
     // iterations past convergence return at once
     const int CHUNK = 2;
@@ -1168,10 +1161,10 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_mgpcg_b<<<G, 256, 0, st>>>(nn, s->g, s->z, w->partials, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
-            rc = mg_vcycle(s->mg, s->g, &sv, st, w->scal);
+            rc = mg_vcycle(s->mg, s->g, &sv, w->partials, st, w->scal);
             if (rc) return rc;
-            k_mgpcg_dot<<<G, 256, 0, st>>>(nn, s->g, sv, w->partials, w->scal);
-            k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 3, nu, Q.tol, it, w->scal);
+            k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, ndot, folded, w->scal);
+            k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 3, nu, Q.tol, it, w->scal);
         }
         PIC_LAUNCH_CHECK();
         rc = read_scalars(w);
