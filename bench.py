@@ -254,16 +254,22 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     out["phi_init"] = phi_init
     # untimed continuation to the comparison horizon; diagnostics of every step
     hist = []
+    late_push = []  # push time of each continuation step (the wake fills in over ~100 steps)
+    w.set_timing(True)
+    w.timers(reset=True)
     for i in range(warmup + steps, max(horizon, warmup + steps)):
         ok, its = step()
         if not ok:
             log(f"[bench]   step {i}: Newton did not converge")
+        late_push.append(w.timers(reset=True)["push"][0])
         d = ions.diag()  # getRealCount, getMomentum (3), getKE
         hist.append([float(ions.np()), d[0], d[3], d[4], w.get_pe()])
         if (i + 1) % 20 == 0:
             log(f"[bench]   {'mixed' if precision == P.MIXED else 'fp64'} step {i + 1}: np {ions.np():.4g}, "
-                f"KE {d[4]:.6e}, PE {hist[-1][4]:.6e}")
+                f"KE {d[4]:.6e}, PE {hist[-1][4]:.6e}, push {np.mean(late_push[-20:]):.2f} ms (mean of last 20)")
+    w.set_timing(False)
     out["history"] = np.array(hist)
+    out["late_push_ms"] = late_push
     out["steps_total"] = max(horizon, warmup + steps)
     if want_fields:
         out["phi"] = w.field(P.PHI)
@@ -425,6 +431,11 @@ def main():
                 "steps": f"{args.warmup + steps + 1}-{mixed['steps_total']}",
                 "particle_count": hdiff(0), "real_count": hdiff(1), "kinetic_energy": hdiff(3),
                 "momentum_z": hdiff(2), "potential_energy": hdiff(4)}
+            # push time late in the run, once the wake behind the sphere has formed
+            line["late_push_ms"] = {
+                "steps": f"{mixed['steps_total'] - 19}-{mixed['steps_total']}",
+                "mixed": float(np.mean(mixed["late_push_ms"][-20:])),
+                "fp64": float(np.mean(fp64["late_push_ms"][-20:]))}
     print(json.dumps(line), flush=True)
 
 

@@ -416,6 +416,35 @@ __device__ inline void deposit_global64(const Grid &g, double *den, int i, int j
 // FP64 (ds_add_f64) for both forms.  Cell-sorted particles make the lanes of a wavefront hit
 // the same window nodes; under that contention ds_add_f32 measured 2.8x slower than
 // ds_add_f64 on gfx950 (44.5 ms vs 16.0 ms for the whole 1e9-particle push).
+// cell and CIC offsets of a particle for a global deposit
+__device__ inline void dep_params(const Grid &g, const PMixed &p, int &i, int &j, int &k, double &di, double &dj,
+                                  double &dk, double &w) {
+    i = p.ci;
+    j = p.cj;
+    k = p.ck;
+    di = p.ox;
+    dj = p.oy;
+    dk = p.oz;
+    w = p.w;
+}
+
+__device__ inline void dep_params(const Grid &g, const PF64 &p, int &i, int &j, int &k, double &di, double &dj,
+                                  double &dk, double &w) {
+    cell_of(g, p, i, j, k, di, dj, dk);
+    w = p.w;
+}
+
+// Global CIC deposit of one particle (outside the LDS windows).  Per-lane atomics: the
+// particles that leave a tile's window in one step mostly land in distinct cells, and
+// grouping equal cells across the wavefront costs more than the atomics it saves.
+template <typename PT>
+__device__ inline void deposit_global(const Grid &g, double *den, const PT &p) {
+    int i, j, k;
+    double di, dj, dk, w;
+    dep_params(g, p, i, j, k, di, dj, dk, w);
+    deposit_global64(g, den, i, j, k, di, dj, dk, w);
+}
+
 template <typename T>
 __device__ inline void deposit_lds(double *sW, int wi, int wj, int wk, T ox, T oy, T oz, T w) {
     int b = wi + WN * (wj + WN * wk);
@@ -435,9 +464,9 @@ __device__ inline void deposit_lds(double *sW, int wi, int wj, int wk, T ox, T o
 // ---------------------------------------------------------------------------------------
 // Tile t gets an own segment of exactly n[t] slots (its input count: every particle that
 // stays fits) followed by an inflow segment for one step of arrivals: the larger of
-// max(n/8, nb/4) (nb the largest face-neighbour population) and 9/8 of the arrivals the tile
+// max(n/8, nb/4) (nb the largest face-neighbour population) and 3/2 of the arrivals the tile
 // saw in the last push (the raw reservation count, including arrivals that found the
-// segment full) plus the staged particles now in the tile, then + 32.  Sizing from observed
+// segment full) plus the staged particles now in the tile, then + 64.  Sizing from observed
 // arrivals lets converging flows (the wake behind the sphere) get the room they need, and
 // counting staged particles drains the staging list instead of re-spilling it.  The inflow
 // room is scaled down uniformly when the buffer is short.  Mode 1 (counting sort) sizes bins
@@ -451,7 +480,10 @@ constexpr int PLAN_SCAN_THREADS = 1024;
 struct PlanTmp {
     uint32_t *need, *slack;
     uint32_t *staged;         // staged particles per current tile (zeroed again by k_plan_caps)
+    uint32_t *nchunk;         // push workgroups per tile
     unsigned long long *blk;  // [2 * nblocks]
+    unsigned long long *blk2; // [nblocks] chunk block sums, then their offsets
+    unsigned long long *total;// [0] push workgroups over all tiles
     double *alpha;            // [0] alpha, [1] 1 when the buffer is short
 };
 
@@ -473,7 +505,7 @@ __device__ inline void plan_sizes(const Grid &g, const PartView &in, const uint3
         if (tz > 0) nb = max(nb, tile_count(in, t - g.ntx * g.nty));
         if (tz < g.ntz - 1) nb = max(nb, tile_count(in, t + g.ntx * g.nty));
         uint32_t arrivals = in.bin_in[t] + staged[t];
-        s = max(max(n / 8, nb / 4), arrivals + arrivals / 8) + 32;
+        s = max(max(n / 8, nb / 4), arrivals + arrivals / 2) + 64;
     } else {
         n = hist[t];
         s = 0;
@@ -559,13 +591,15 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_caps(Grid g, PlanTmp T, P
     if (threadIdx.x == 0) T.blk[blockIdx.x] = a;
 }
 
-// exclusive scan of the nblk block sums in place (one workgroup, fixed order)
-__global__ void __launch_bounds__(PLAN_SCAN_THREADS) k_plan_scan(PlanTmp T, int nblk) {
+// exclusive scan of the nblk block sums in blk, in place (one workgroup, fixed order);
+// total (nullable) receives their sum
+__global__ void __launch_bounds__(PLAN_SCAN_THREADS)
+    k_plan_scan(unsigned long long *blk, int nblk, unsigned long long *total) {
     __shared__ unsigned long long s_scan[PLAN_SCAN_THREADS];
     int per = (nblk + PLAN_SCAN_THREADS - 1) / PLAN_SCAN_THREADS;
     int b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
     unsigned long long local = 0;
-    for (int b = b0; b < b1; b++) local += T.blk[b];
+    for (int b = b0; b < b1; b++) local += blk[b];
     s_scan[threadIdx.x] = local;
     __syncthreads();
     for (int off = 1; off < PLAN_SCAN_THREADS; off <<= 1) {
@@ -576,10 +610,11 @@ __global__ void __launch_bounds__(PLAN_SCAN_THREADS) k_plan_scan(PlanTmp T, int 
     }
     unsigned long long run = s_scan[threadIdx.x] - local;
     for (int b = b0; b < b1; b++) {
-        unsigned long long v = T.blk[b];
-        T.blk[b] = run;
+        unsigned long long v = blk[b];
+        blk[b] = run;
         run += v;
     }
+    if (total && threadIdx.x == PLAN_SCAN_THREADS - 1) *total = s_scan[threadIdx.x];
 }
 
 __global__ void __launch_bounds__(PLAN_THREADS) k_plan_starts(Grid g, PlanTmp T, PartView out) {
@@ -602,10 +637,46 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_starts(Grid g, PlanTmp T,
     if (t == 0) *out.stage_count = 0;
 }
 
+// push workgroups per tile: one per PUSH_CHUNK of its input particles, at least one
+__global__ void __launch_bounds__(PLAN_THREADS) k_plan_chunks(Grid g, PlanTmp T) {
+    __shared__ unsigned long long sh[PLAN_THREADS / 64];
+    int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
+    uint32_t c = 0;
+    if (t < g.ntiles) {
+        c = max(1u, (T.need[t] + PUSH_CHUNK - 1) / PUSH_CHUNK);
+        T.nchunk[t] = c;
+    }
+    unsigned long long a = block_sum_u64(c, sh);
+    if (threadIdx.x == 0) T.blk2[blockIdx.x] = a;
+}
+
+// map[w] = tile | chunk << 32 for push workgroup w, tiles in order; entries past the map's
+// capacity (only when the particle count exceeds the bins, which the plan flags) are dropped
+__global__ void __launch_bounds__(PLAN_THREADS) k_plan_map(Grid g, PlanTmp T, uint64_t *map, long cap) {
+    __shared__ unsigned long long s_scan[PLAN_THREADS];
+    int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
+    unsigned long long c = t < g.ntiles ? T.nchunk[t] : 0;
+    s_scan[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < PLAN_THREADS; off <<= 1) {
+        unsigned long long v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (t < g.ntiles) {
+        unsigned long long w0 = T.blk2[blockIdx.x] + s_scan[threadIdx.x] - c;
+        for (unsigned long long k = 0; k < c && w0 + k < (unsigned long long)cap; k++)
+            map[w0 + k] = (uint64_t)t | ((uint64_t)k << 32);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // the fused push: gather (LDS E tile) + leapfrog + walls + CIC deposit (LDS window) + rebin
 // ---------------------------------------------------------------------------------------
 struct PushOut {
+    const uint64_t *chunk_map;           // workgroup -> tile | chunk << 32
+    const unsigned long long *total_groups;  // workgroups the map holds
     float *win32;
     double *win64;
     uint8_t *win_used;
@@ -666,12 +737,29 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         acc[5] += 1;
     };
 
-    if ((int)blockIdx.x < g.ntiles) {
+    const unsigned nbin = gridDim.x - PUSH_STAGE_WGS;  // workgroups for the binned tiles
+    if (blockIdx.x < nbin) {
         // ------------------------------------------------ binned tile: waves run independently
-        int t = tile_of_block(blockIdx.x, g.ntiles, P.remap);
+        // workgroup -> (tile, chunk) from the plan's map.  Workgroups are dispatched
+        // round-robin over the 8 XCDs (XCD b % 8); the j-th one of an XCD takes entry
+        // (j / S * 8 + xcd) * S + j % S, S = PUSH_XCD_STRIPE: a bijection of [0, nbin)
+        // (nbin is a multiple of 8 S) that deals out stripes of S entries round-robin
+        const unsigned total = (unsigned)min(*O.total_groups, (unsigned long long)nbin);
+        const unsigned xcd = blockIdx.x % 8u, j = blockIdx.x / 8u;
+        const unsigned wg = ((j / PUSH_XCD_STRIPE) * 8u + xcd) * PUSH_XCD_STRIPE + j % PUSH_XCD_STRIPE;
+        if (wg >= total) {
+            if (tid < NDIAG_SLOTS) O.diag_part[tid * gridDim.x + blockIdx.x] = 0.0;
+            return;
+        }
+        const uint64_t tc = O.chunk_map[wg];
+        const int t = (int)(uint32_t)tc;
+        const uint32_t chunk = (uint32_t)(tc >> 32);
         BinSeg bs = bin_of(in, t);
-        uint32_t cnt = bs.c0 + bs.c1;
-        if (cnt == 0) {
+        const uint32_t cnt_tile = bs.c0 + bs.c1;
+        const bool split = cnt_tile > (uint32_t)PUSH_CHUNK;  // several workgroups share tile t
+        const uint32_t q_lo = chunk * (uint32_t)PUSH_CHUNK;
+        const uint32_t q_hi = min(cnt_tile, q_lo + (uint32_t)PUSH_CHUNK);
+        if (q_lo >= q_hi) {
             if (tid == 0) O.win_used[t] = 0;
             if (tid < NDIAG_SLOTS) O.diag_part[tid * gridDim.x + blockIdx.x] = 0.0;
             return;
@@ -714,14 +802,14 @@ __global__ void __launch_bounds__(PUSH_THREADS)
 
         // the next particle of each thread is loaded while the current one is processed
         PT pn;
-        if (tid < cnt) load_p(in, seg_index(bs, tid), pn);
-        for (uint32_t base = 0; base < cnt; base += PUSH_THREADS) {
+        if (q_lo + tid < q_hi) load_p(in, seg_index(bs, q_lo + tid), pn);
+        for (uint32_t base = q_lo; base < q_hi; base += PUSH_THREADS) {
             uint32_t q = base + tid;
             PT p = pn;
-            if (q + PUSH_THREADS < cnt) load_p(in, seg_index(bs, q + PUSH_THREADS), pn);
-            bool alive = false;
+            if (q + PUSH_THREADS < q_hi) load_p(in, seg_index(bs, q + PUSH_THREADS), pn);
+            bool alive = false, far = false;  // far: deposits outside this tile's window
             int dtile = -1, nbi = -1;  // destination tile, and its index among the 27 around t
-            if (q < cnt) {
+            if (q < q_hi) {
                 int ci, cj, ck;  // cell after the push
                 if constexpr (PREC == PIC_MIXED) {
                     int li = min(max(p.ci - c0x, 0), TILE - 1);
@@ -738,8 +826,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                             wk < TILE + 2 * HALO)
                             deposit_lds<float>(sW, wi, wj, wk, p.ox, p.oy, p.oz, p.w);
                         else
-                            deposit_global64(g, O.den_extra, p.ci, p.cj, p.ck, (double)p.ox, (double)p.oy,
-                                             (double)p.oz, (double)p.w);
+                            far = true;
                     }
                 } else {
                     double di, dj, dk;
@@ -758,7 +845,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                             wk < TILE + 2 * HALO)
                             deposit_lds<double>(sW, wi, wj, wk, di, dj, dk, p.w);
                         else
-                            deposit_global64(g, O.den_extra, ci, cj, ck, di, dj, dk, p.w);
+                            far = true;
                     }
                 }
                 if (alive) {
@@ -769,6 +856,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                         nbi = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
                 }
             }
+            if (far) deposit_global(g, O.den_extra, p);
             // stayers: one LDS atomic per wavefront reserves their slots in the own segment
             bool stay = alive && dtile == t;
             long idx = -1;
@@ -778,7 +866,9 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                 int leader = __ffsll((long long)m) - 1;
                 uint32_t wbase = 0;
-                if ((int)__lane_id() == leader) wbase = atomicAdd(&sOwn, (uint32_t)__popcll(m));
+                if ((int)__lane_id() == leader)
+                    wbase = split ? atomicAdd(&out.bin_own[t], (uint32_t)__popcll(m))
+                                  : atomicAdd(&sOwn, (uint32_t)__popcll(m));
                 wbase = __shfl(wbase, leader, 64);
                 uint32_t slot = wbase + before;
                 if (stay && slot < own_cap) idx = own0 + slot;
@@ -800,39 +890,46 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             if (idx >= 0) store_p(out, idx, p);
         }
         __syncthreads();
-        // flush the deposit window of this tile
-        if (PREC == PIC_MIXED) {
-            float *dst = O.win32 + (long)t * WNODES;
-            for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = (float)sW[n];
+        if (chunk == 0) {
+            // flush the deposit window into the tile's window slot
+            if (PREC == PIC_MIXED) {
+                float *dst = O.win32 + (long)t * WNODES;
+                for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = (float)sW[n];
+            } else {
+                double *dst = O.win64 + (long)t * WNODES;
+                for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = sW[n];
+            }
+            if (tid == 0) {
+                O.win_used[t] = 1;
+                if (!split) out.bin_own[t] = sOwn;
+            }
         } else {
-            double *dst = O.win64 + (long)t * WNODES;
-            for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = sW[n];
-        }
-        if (tid == 0) {
-            O.win_used[t] = 1;
-            out.bin_own[t] = sOwn;
+            // further chunks of a split tile add their window to the far-deposit array
+            for (int n = tid; n < WNODES; n += PUSH_THREADS) {
+                double v = sW[n];
+                if (v == 0.0) continue;
+                int gi = c0x - HALO + n % WN, gj = c0y - HALO + (n / WN) % WN, gk = c0z - HALO + n / (WN * WN);
+                if (gi >= 0 && gi < g.ni && gj >= 0 && gj < g.nj && gk >= 0 && gk < g.nk)
+                    atomicAdd(&O.den_extra[gi + sj * gj + sk * gk], v);
+            }
         }
     } else {
         // ------------------------------------------------ unbinned staging list
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
-        uint32_t nthreads = (gridDim.x - g.ntiles) * PUSH_THREADS;
-        uint32_t first = (blockIdx.x - g.ntiles) * PUSH_THREADS + tid;
+        uint32_t nthreads = (gridDim.x - nbin) * PUSH_THREADS;
+        uint32_t first = (blockIdx.x - nbin) * PUSH_THREADS + tid;
         uint32_t trips = (ns + nthreads - 1) / nthreads;  // uniform per wavefront
         for (uint32_t it = 0; it < trips; it++) {
             uint32_t q = first + it * nthreads;
             PT p;
             int dt = -1;
+            bool alive = false;
             if (q < ns) {
                 load_p(in, in.cap_bins + q, p);
-                bool alive;
                 if constexpr (PREC == PIC_MIXED) {
                     float3 E = gather_global32(g, ef32, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz);
                     alive = push_mixed(P, p, E);
-                    if (alive) {
-                        deposit_global64(g, O.den_extra, p.ci, p.cj, p.ck, (double)p.ox, (double)p.oy,
-                                         (double)p.oz, (double)p.w);
-                        dt = tile_of(g, p);
-                    }
+                    if (alive) dt = tile_of(g, p);
                 } else {
                     int i, j, k;
                     double di, dj, dk;
@@ -842,12 +939,12 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                     alive = push_f64(P, p, E);
                     if (alive) {
                         cell_of(g, p, i, j, k, di, dj, dk);
-                        deposit_global64(g, O.den_extra, i, j, k, di, dj, dk, p.w);
                         dt = tile_id(g, i, j, k);
                     }
                 }
                 if (alive) tally(p);
             }
+            if (alive) deposit_global(g, O.den_extra, p);
             uint32_t pos = wave_reserve_global(dt, out.bin_in);
             if (dt >= 0) store_inflow(out, p, dt, pos, O.err);
         }
@@ -901,34 +998,24 @@ __global__ void __launch_bounds__(256)
     den[u] = (V != 0) ? raw / V : 0.0;
 }
 
-// deposit-only pass (computeNumberDensity without a preceding push): global FP64 atomics.
+// deposit-only pass (computeNumberDensity without a preceding push): global FP64 atomics
 template <int PREC>
 __global__ void __launch_bounds__(256) k_deposit_all(Grid g, PartView in, double *den_extra) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
-    auto dep = [&](const PT &p) {
-        if constexpr (PREC == PIC_MIXED) {
-            deposit_global64(g, den_extra, p.ci, p.cj, p.ck, p.ox, p.oy, p.oz, p.w);
-        } else {
-            int i, j, k;
-            double di, dj, dk;
-            cell_of(g, p, i, j, k, di, dj, dk);
-            deposit_global64(g, den_extra, i, j, k, di, dj, dk, p.w);
-        }
-    };
     int b = blockIdx.x;
     if (b < g.ntiles) {
         BinSeg bs = bin_of(in, b);
         for (uint32_t q = threadIdx.x; q < bs.c0 + bs.c1; q += blockDim.x) {
             PT p;
             load_p(in, seg_index(bs, q), p);
-            dep(p);
+            deposit_global(g, den_extra, p);
         }
     } else {
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
         for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns; q += (gridDim.x - g.ntiles) * blockDim.x) {
             PT p;
             load_p(in, in.cap_bins + q, p);
-            dep(p);
+            deposit_global(g, den_extra, p);
         }
     }
 }
@@ -1359,17 +1446,49 @@ static PlanTmp plan_tmp(pic_species *s) {
     size_t nt = (size_t)g.ntiles;
     char *p = (char *)s->plan_mem;
     PlanTmp T;
+    size_t nblk = (nt + PLAN_THREADS - 1) / PLAN_THREADS;
     T.need = (uint32_t *)p;
     T.slack = (uint32_t *)(p + 4 * nt);
     T.staged = (uint32_t *)(p + 8 * nt);
-    T.alpha = (double *)(p + ((12 * nt + 7) & ~(size_t)7));
-    T.blk = (unsigned long long *)((char *)T.alpha + 16);
+    T.nchunk = (uint32_t *)(p + 12 * nt);
+    T.alpha = (double *)(p + ((16 * nt + 7) & ~(size_t)7));
+    T.total = (unsigned long long *)((char *)T.alpha + 16);
+    T.blk = T.total + 1;
+    T.blk2 = T.blk + 2 * nblk;
     return T;
 }
 
 static size_t plan_bytes(const Grid &g) {
     size_t nblk = ((size_t)g.ntiles + PLAN_THREADS - 1) / PLAN_THREADS;
-    return ((12 * (size_t)g.ntiles + 7) & ~(size_t)7) + 16 + 16 * nblk;
+    return ((16 * (size_t)g.ntiles + 7) & ~(size_t)7) + 16 + 8 + 24 * nblk;
+}
+
+// push workgroups over the tiles: at most one per tile plus one per PUSH_CHUNK bin slots,
+// padded to whole rounds of XCD stripes
+static long push_bin_groups(const Grid &g, long cap_bins) {
+    const long r = 8L * PUSH_XCD_STRIPE;
+    long n = (long)g.ntiles + cap_bins / PUSH_CHUNK + 1;
+    return (n + r - 1) / r * r;
+}
+
+// chunk map and per-workgroup diagnostic partials sized for the buffers' bin capacity
+static int ensure_push_scratch(pic_species *s) {
+    const Grid &g = s->w->g;
+    long nb = push_bin_groups(g, s->buf[1 - s->cur].cap_bins);
+    if (nb > s->chunk_cap) {
+        if (s->chunk_map) PIC_HIP(hipFree(s->chunk_map));
+        s->chunk_map = nullptr;
+        PIC_HIP(hipMalloc(&s->chunk_map, sizeof(uint64_t) * (size_t)nb));
+        s->chunk_cap = nb;
+    }
+    long nd = std::max<long>(nb + PUSH_STAGE_WGS, g.ntiles + STAGE_WGS);
+    if (nd > s->diag_cap) {
+        if (s->diag_part) PIC_HIP(hipFree(s->diag_part));
+        s->diag_part = nullptr;
+        PIC_HIP(hipMalloc(&s->diag_part, sizeof(double) * NDIAG_SLOTS * (size_t)nd));
+        s->diag_cap = nd;
+    }
+    return PIC_OK;
 }
 
 // segment plan of out from the counts of in (mode 0) or from the histogram hist (mode 1)
@@ -1389,8 +1508,13 @@ static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist,
     k_plan_count<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, in, hist, mode, T);
     k_plan_total<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T, nblk, (unsigned long long)out.cap_bins, err);
     k_plan_caps<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, vo);
-    k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T, nblk);
+    k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk, nblk, nullptr);
     k_plan_starts<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, vo);
+    if (mode == 0) {  // push workgroups per tile
+        k_plan_chunks<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T);
+        k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk2, nblk, T.total);
+        k_plan_map<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->chunk_map, s->chunk_cap);
+    }
     PIC_LAUNCH_CHECK();
     return PIC_OK;
 }
@@ -1548,7 +1672,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMalloc(&s->win_used, g.ntiles));
     TRY(hipMalloc(&s->plan_mem, plan_bytes(g)));
     TRY(hipMalloc(&s->diag_dev, sizeof(double) * 8));
-    TRY(hipMalloc(&s->diag_part, sizeof(double) * NDIAG_SLOTS * (size_t)(g.ntiles + STAGE_WGS)));
+    if ((rc = ensure_push_scratch(s))) goto fail;
     TRY(hipHostMalloc(&s->diag_host, sizeof(double) * 8));
     TRY(hipMemsetAsync(s->np_dev, 0, sizeof(long), w->stream));
     TRY(hipMemsetAsync(s->den, 0, sizeof(double) * g.nn, w->stream));
@@ -1584,6 +1708,7 @@ int pic_species_destroy(pic_species *s) {
     hipFree(s->plan_mem);
     hipFree(s->diag_dev);
     hipFree(s->diag_part);
+    hipFree(s->chunk_map);
     if (s->diag_host) hipHostFree(s->diag_host);
     delete s;
     return PIC_OK;
@@ -1776,6 +1901,8 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
         if (rc) return rc;
     }
     PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
+    rc = ensure_push_scratch(s);
+    if (rc) return rc;
     // the push phase: segment plan of the output buffer, then the fused push
     hipEvent_t ev;
     timer_begin(w, PIC_T_PUSH, &ev);
@@ -1801,13 +1928,17 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     P.r2f = (float)w->sphere.r2;
     P.sph_r2 = w->sphere.r2;
     PushOut O{};
+    O.chunk_map = s->chunk_map;
+    O.total_groups = plan_tmp(s).total;
     O.win32 = (float *)s->win;
     O.win64 = (double *)s->win;
     O.win_used = s->win_used;
     O.den_extra = s->den_extra;
     O.diag_part = s->diag_part;
     O.err = err_flag(s);
-    unsigned grid = g.ntiles + STAGE_WGS;
+    // workgroups for the tiles (the plan's map decides which run) + the staging workgroups
+    unsigned grid = (unsigned)(push_bin_groups(g, out.cap_bins) + PUSH_STAGE_WGS);
+    s->diag_nb = (int)grid;
     if (s->prec == PIC_MIXED)
         k_push<PIC_MIXED><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
     else
@@ -1876,8 +2007,8 @@ int pic_species_update_averages(pic_species *s) {
 int pic_species_diag(pic_species *s, double out[5]) {
     PIC_CHECK(s && out, PIC_ERR_ARG, "null argument");
     pic_world *w = s->w;
-    unsigned grid = w->g.ntiles + STAGE_WGS;
     if (!s->diag_valid) {
+        unsigned grid = w->g.ntiles + STAGE_WGS;
         PartView v = view_of(s->buf[s->cur]);
         if (s->prec == PIC_MIXED)
             k_diag_all<PIC_MIXED><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_part);
@@ -1885,8 +2016,9 @@ int pic_species_diag(pic_species *s, double out[5]) {
             k_diag_all<PIC_FP64><<<grid, 256, 0, w->stream>>>(w->g, v, s->diag_part);
         PIC_LAUNCH_CHECK();
         s->diag_valid = true;
+        s->diag_nb = (int)grid;
     }
-    k_reduce_slots<<<1, 1024, 0, w->stream>>>(s->diag_part, (int)grid, NDIAG_SLOTS, s->diag_dev);
+    k_reduce_slots<<<1, 1024, 0, w->stream>>>(s->diag_part, s->diag_nb, NDIAG_SLOTS, s->diag_dev);
     PIC_LAUNCH_CHECK();
     PIC_HIP(hipMemcpyAsync(s->diag_host, s->diag_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, w->stream));
     PIC_HIP(hipStreamSynchronize(w->stream));

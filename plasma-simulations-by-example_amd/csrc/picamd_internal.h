@@ -36,7 +36,18 @@ constexpr int PUSH_THREADS = 256;
 constexpr int PUSH_PER_THREAD = 4;        // particles per thread per batch
 constexpr int PUSH_BATCH = PUSH_THREADS * PUSH_PER_THREAD;
 constexpr int NDEST = 27;                 // destination tiles reachable in one step (3x3x3)
-constexpr int STAGE_WGS = 64;             // workgroups that push the unbinned list
+constexpr int STAGE_WGS = 64;             // workgroups that walk the unbinned list (sort, diag)
+// The push gives each tile one workgroup per PUSH_CHUNK of its particles: flows that pile
+// ions into a few tiles (the focused wake behind the sphere reaches 40x the mean tile
+// population within 100 steps) would otherwise leave one workgroup running long after the
+// rest.  PUSH_STAGE_WGS workgroups push the unbinned list.
+constexpr int PUSH_CHUNK = 32768;
+constexpr int PUSH_STAGE_WGS = 256;
+// Push workgroups are handed to the 8 XCDs in stripes of PUSH_XCD_STRIPE consecutive
+// (tile, chunk) entries, round-robin: each XCD's L2 serves runs of neighbouring tiles, and
+// every XCD gets a share of each region of the domain.  (Contiguous eighths per XCD left
+// the XCDs that own the wake running at twice the others' time.)
+constexpr int PUSH_XCD_STRIPE = 32;
 
 // packed cell index of PIC_MIXED particles: i | j<<10 | k<<20
 constexpr int CELL_BITS_I = 10, CELL_BITS_J = 10, CELL_BITS_K = 11;
@@ -176,6 +187,10 @@ struct pic_species {
     void *win = nullptr;          // per-tile deposit windows (float or double)
     uint8_t *win_used = nullptr;
     void *plan_mem = nullptr;     // scratch of the segment plan (per-tile counts, block sums)
+    uint64_t *chunk_map = nullptr;  // push workgroup -> (tile, chunk), written by the plan
+    long chunk_cap = 0;           // entries of chunk_map
+    long diag_cap = 0;            // workgroups diag_part has room for
+    int diag_nb = 0;              // workgroups whose partials diag_part holds
     double *diag_dev = nullptr;   // [6] reduced diagnostics
     double *diag_part = nullptr;  // per-block partials of the last push (or diag pass)
     double *diag_host = nullptr;  // pinned
