@@ -67,7 +67,7 @@ __device__ inline KeyGroup wave_group(int key) {
     unsigned long long active = __ballot(key >= 0);
     while (active) {
         int leader = __ffsll((long long)active) - 1;
-        int lk = __shfl(key, leader, 64);
+        int lk = __builtin_amdgcn_readlane(key, leader);
         unsigned long long m = __ballot(key == lk);
         if (key == lk) {
             kg.leader = leader;
@@ -82,13 +82,17 @@ __device__ inline KeyGroup wave_group(int key) {
 // Reserves counter[key] slots for the lanes of each distinct key: the leaders of all keys
 // issue their atomics in one instruction (one round trip per wavefront, not one per key),
 // then each lane reads its leader's base.  Returns the lane's slot (0 for key < 0).
+__device__ inline uint32_t wave_group_base(const KeyGroup &kg, int key, uint32_t leader_base) {
+    int src = kg.leader < 0 ? (int)__lane_id() : kg.leader;
+    uint32_t base = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)leader_base);
+    return key >= 0 ? base + kg.before : 0u;
+}
+
 __device__ inline uint32_t wave_reserve_global(int key, uint32_t *counter) {
     KeyGroup kg = wave_group(key);
     uint32_t base = 0;
     if (key >= 0 && kg.before == 0) base = atomicAdd(&counter[key], kg.count);
-    int src = kg.leader < 0 ? (int)__lane_id() : kg.leader;
-    base = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)base);
-    return key >= 0 ? base + kg.before : 0u;
+    return wave_group_base(kg, key, base);
 }
 
 __device__ inline unsigned long long wave_count_u32(int key, uint32_t *counter) {
@@ -96,7 +100,7 @@ __device__ inline unsigned long long wave_count_u32(int key, uint32_t *counter) 
     unsigned long long active = __ballot(key >= 0);
     while (active) {
         int leader = __ffsll((long long)active) - 1;
-        int lk = __shfl(key, leader, 64);
+        int lk = __builtin_amdgcn_readlane(key, leader);
         unsigned long long m = __ballot(key == lk);
         if ((int)__lane_id() == leader) atomicAdd(&counter[lk], (uint32_t)__popcll(m));
         active &= ~m;
@@ -691,20 +695,20 @@ __device__ inline long inflow_slot(const PartView &out, int dtile, uint32_t pos)
     return pos < cap - off ? (long)out.bin_start[dtile] + off + pos : -1L;
 }
 
-// slot in the staging list, -1 (and a capacity error) when it is full
-__device__ inline long staging_slot(const PartView &out, uint32_t *err) {
-    uint32_t s = atomicAdd(out.stage_count, 1u);
+// slots in the staging list for the lanes with `need` (one atomic per wavefront); -1, and a
+// capacity error, when the list is full.  Call from wavefront-uniform control flow.
+__device__ inline long staging_slot(bool need, const PartView &out, uint32_t *err) {
+    const unsigned long long m = __ballot(need);
+    if (!m) return -1L;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(out.stage_count, (uint32_t)__popcll(m));
+    const uint32_t s = (uint32_t)__builtin_amdgcn_readlane((int)base, leader) +
+                       __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+    if (!need) return -1L;
     if (s < out.cap_stage) return out.cap_bins + s;
     atomicOr(err, 2u);  // staging full: particle dropped, reported as a capacity error
     return -1L;
-}
-
-// slot of an arrival in the inflow segment of tile dtile, or the staging list when full
-template <typename P>
-__device__ inline void store_inflow(const PartView &out, const P &p, int dtile, uint32_t pos, uint32_t *err) {
-    long idx = inflow_slot(out, dtile, pos);
-    if (idx < 0) idx = staging_slot(out, err);
-    if (idx >= 0) store_p(out, idx, p);
 }
 
 template <int PREC>
@@ -807,8 +811,10 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             uint32_t q = base + tid;
             PT p = pn;
             if (q + PUSH_THREADS < q_hi) load_p(in, seg_index(bs, q + PUSH_THREADS), pn);
-            bool alive = false, far = false;  // far: deposits outside this tile's window
-            int dtile = -1, nbi = -1;  // destination tile, and its index among the 27 around t
+            bool alive = false;
+            int dtile = -1, nbi = -1;   // destination tile, and its index among the 27 around t
+            int wi = -1, wj = 0, wk = 0;  // deposit window cell; wi < 0: outside the window
+            double di = 0, dj = 0, dk = 0;  // FP64 form: CIC offsets after the push
             if (q < q_hi) {
                 int ci, cj, ck;  // cell after the push
                 if constexpr (PREC == PIC_MIXED) {
@@ -820,16 +826,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                     ci = p.ci;
                     cj = p.cj;
                     ck = p.ck;
-                    if (alive) {
-                        int wi = p.ci - c0x + HALO, wj = p.cj - c0y + HALO, wk = p.ck - c0z + HALO;
-                        if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO && wk >= 0 &&
-                            wk < TILE + 2 * HALO)
-                            deposit_lds<float>(sW, wi, wj, wk, p.ox, p.oy, p.oz, p.w);
-                        else
-                            far = true;
-                    }
                 } else {
-                    double di, dj, dk;
                     cell_of(g, p, ci, cj, ck, di, dj, dk);
                     int li = min(max(ci - c0x, 0), TILE - 1);
                     int lj = min(max(cj - c0y, 0), TILE - 1);
@@ -838,45 +835,53 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                     double3 E = gather64_weights(pl, pl + TNODES, pl + 2 * TNODES, li + TN * (lj + TN * lk), TN,
                                                  TN * TN, di, dj, dk);
                     alive = push_f64(P, p, E);
-                    if (alive) {
-                        cell_of(g, p, ci, cj, ck, di, dj, dk);
-                        int wi = ci - c0x + HALO, wj = cj - c0y + HALO, wk = ck - c0z + HALO;
-                        if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO && wk >= 0 &&
-                            wk < TILE + 2 * HALO)
-                            deposit_lds<double>(sW, wi, wj, wk, di, dj, dk, p.w);
-                        else
-                            far = true;
-                    }
+                    if (alive) cell_of(g, p, ci, cj, ck, di, dj, dk);
                 }
                 if (alive) {
                     tally(p);
+                    int a = ci - c0x + HALO, b = cj - c0y + HALO, c = ck - c0z + HALO;
+                    if (a >= 0 && a < TILE + 2 * HALO && b >= 0 && b < TILE + 2 * HALO && c >= 0 &&
+                        c < TILE + 2 * HALO) {
+                        wi = a;
+                        wj = b;
+                        wk = c;
+                    }
                     dtile = tile_id(g, ci, cj, ck);
                     int dx = (ci >> TILE_SHIFT) - tx, dy = (cj >> TILE_SHIFT) - ty, dz = (ck >> TILE_SHIFT) - tz;
                     if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
                         nbi = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
                 }
             }
-            if (far) deposit_global(g, O.den_extra, p);
-            // stayers: one LDS atomic per wavefront reserves their slots in the own segment
-            bool stay = alive && dtile == t;
+            // Slots first, deposit second: the reservation atomics are in flight while the
+            // deposit runs.  Stayers: one atomic per wavefront on the own segment's counter
+            // (LDS, or global when the tile is split); arrivals elsewhere: one atomic per
+            // distinct destination, all in one instruction.
+            const bool stay = alive && dtile == t;
+            const unsigned long long m = __ballot(stay);
+            const int sleader = m ? __ffsll((long long)m) - 1 : 0;
+            uint32_t sbase = 0;
+            if (m && (int)__lane_id() == sleader)
+                sbase = split ? atomicAdd(&out.bin_own[t], (uint32_t)__popcll(m))
+                              : atomicAdd(&sOwn, (uint32_t)__popcll(m));
+            const int key = (alive && !stay) ? dtile : -1;
+            const KeyGroup kg = wave_group(key);
+            uint32_t lbase = 0;
+            if (key >= 0 && kg.before == 0) lbase = atomicAdd(&out.bin_in[key], kg.count);
+            if (wi >= 0) {
+                if constexpr (PREC == PIC_MIXED)
+                    deposit_lds<float>(sW, wi, wj, wk, p.ox, p.oy, p.oz, p.w);
+                else
+                    deposit_lds<double>(sW, wi, wj, wk, di, dj, dk, p.w);
+            } else if (alive) {
+                deposit_global(g, O.den_extra, p);
+            }
             long idx = -1;
-            unsigned long long m = __ballot(stay);
             if (m) {
-                uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                int leader = __ffsll((long long)m) - 1;
-                uint32_t wbase = 0;
-                if ((int)__lane_id() == leader)
-                    wbase = split ? atomicAdd(&out.bin_own[t], (uint32_t)__popcll(m))
-                                  : atomicAdd(&sOwn, (uint32_t)__popcll(m));
-                wbase = __shfl(wbase, leader, 64);
-                uint32_t slot = wbase + before;
+                uint32_t slot = (uint32_t)__builtin_amdgcn_readlane((int)sbase, sleader) +
+                                __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                 if (stay && slot < own_cap) idx = own0 + slot;
             }
-            // arrivals elsewhere: wavefront-aggregated reservation in the destination's
-            // inflow segment
-            int key = (alive && !stay) ? dtile : -1;
-            uint32_t pos = wave_reserve_global(key, out.bin_in);
+            uint32_t pos = wave_group_base(kg, key, lbase);
             if (key >= 0) {
                 if (nbi >= 0) {
                     uint32_t off = sNb[1][nbi], cap = sNb[2][nbi];
@@ -886,7 +891,9 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                 }
             }
             // no room (bins shrunk by a short buffer, inflow segment full): staging list
-            if (alive && idx < 0) idx = staging_slot(out, O.err);
+            const bool to_stage = alive && idx < 0;
+            const long sidx = staging_slot(to_stage, out, O.err);
+            if (to_stage) idx = sidx;
             if (idx >= 0) store_p(out, idx, p);
         }
         __syncthreads();
@@ -946,7 +953,11 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             }
             if (alive) deposit_global(g, O.den_extra, p);
             uint32_t pos = wave_reserve_global(dt, out.bin_in);
-            if (dt >= 0) store_inflow(out, p, dt, pos, O.err);
+            long idx = dt >= 0 ? inflow_slot(out, dt, pos) : -1L;
+            const bool to_stage = dt >= 0 && idx < 0;
+            const long sidx = staging_slot(to_stage, out, O.err);
+            if (to_stage) idx = sidx;
+            if (idx >= 0) store_p(out, idx, p);
         }
     }
     // per-block partials of getRealCount, getMomentum, getKE and the particle count
