@@ -80,7 +80,8 @@ extern "C" {
 #define PIC_T_EF 4      /* computeEF */
 #define PIC_T_RHO 5     /* computeChargeDensity */
 #define PIC_T_INJECT 6  /* device injection */
-#define PIC_T_COUNT 7
+#define PIC_T_PUSH_KERNEL 7 /* the fused push kernel alone (PIC_T_PUSH less the segment plan) */
+#define PIC_T_COUNT 8
 
 typedef struct pic_world pic_world;
 typedef struct pic_species pic_species;

@@ -29,7 +29,6 @@ struct PushParams {
     Grid g;
     int bc;
     int sphere_on;
-    int remap;  // XCD-aware block -> tile remap
     // PIC_MIXED (FP32, cell-relative units)
     float qm_dt;     // dt*q/m
     float dt_dh[3];  // dt/dh
@@ -45,15 +44,6 @@ struct PushParams {
 };
 
 __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
-
-// block index -> tile index.  Blocks are dispatched round-robin over the 8 XCDs; giving
-// each XCD a contiguous run of tiles keeps neighbouring E tiles in the same L2.  Only used
-// when ntiles is a multiple of 8 (then it is a bijection on [0, ntiles)).
-__device__ inline int tile_of_block(int b, int ntiles, int remap) {
-    if (!remap) return b;
-    int per = ntiles >> 3;
-    return (b & 7) * per + (b >> 3);
-}
 
 // Groups the lanes of a wavefront by key (keys >= 0) with ballots only: for each lane, the
 // lowest lane holding its key (its leader), its rank among those lanes and their number.
@@ -1923,7 +1913,6 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     P.g = g;
     P.bc = bc;
     P.sphere_on = (bc == PIC_BC_ABSORB) && w->sphere.on;
-    P.remap = (g.ntiles % 8 == 0 && g.ntiles >= 64) ? 1 : 0;
     P.qm_dt = (float)(dt * s->charge / s->mass);
     P.s64 = dt * s->charge / s->mass;
     P.dt = dt;
@@ -1950,11 +1939,14 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     // workgroups for the tiles (the plan's map decides which run) + the staging workgroups
     unsigned grid = (unsigned)(push_bin_groups(g, out.cap_bins) + PUSH_STAGE_WGS);
     s->diag_nb = (int)grid;
+    hipEvent_t evk;
+    timer_begin(w, PIC_T_PUSH_KERNEL, &evk);
     if (s->prec == PIC_MIXED)
         k_push<PIC_MIXED><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
     else
         k_push<PIC_FP64><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
     PIC_LAUNCH_CHECK();
+    timer_end(w, PIC_T_PUSH_KERNEL, evk);
     timer_end(w, PIC_T_PUSH, ev);
     s->cur = 1 - s->cur;
     s->den_state = 1;

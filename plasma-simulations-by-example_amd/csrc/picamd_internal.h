@@ -33,8 +33,6 @@ constexpr int HALO = 2;
 constexpr int WN = TILE + 1 + 2 * HALO;   // window nodes per edge
 constexpr int WNODES = WN * WN * WN;      // 2197
 constexpr int PUSH_THREADS = 256;
-constexpr int PUSH_PER_THREAD = 4;        // particles per thread per batch
-constexpr int PUSH_BATCH = PUSH_THREADS * PUSH_PER_THREAD;
 constexpr int NDEST = 27;                 // destination tiles reachable in one step (3x3x3)
 constexpr int STAGE_WGS = 64;             // workgroups that walk the unbinned list (sort, diag)
 // The push gives each tile one workgroup per PUSH_CHUNK of its particles: flows that pile

@@ -23,7 +23,7 @@ REFLECT, ABSORB = 0, 1
 GS, PCG, QN, SOR_LINEAR, PCG_SYM, MG = 0, 1, 2, 3, 4, 5
 PHI, RHO, NODE_VOL, EF, OBJECT_ID = 0, 1, 2, 3, 4
 T_PUSH, T_DEPOSIT, T_SORT, T_SOLVE, T_EF, T_RHO, T_INJECT = range(7)
-T_NAMES = ["push", "deposit", "sort", "solve", "ef", "rho", "inject"]
+T_NAMES = ["push", "deposit", "sort", "solve", "ef", "rho", "inject", "push_kernel"]
 
 # every symbol include/picamd.h declares
 EXPORTS = [
@@ -219,11 +219,11 @@ class World:
         _check(lib().pic_world_set_timing(self.h, 1 if on else 0))
 
     def timers(self, reset=False):
-        ms = np.zeros(7)
-        calls = np.zeros(7, dtype=np.int64)
+        ms = np.zeros(len(T_NAMES))
+        calls = np.zeros(len(T_NAMES), dtype=np.int64)
         _check(lib().pic_world_get_timers(self.h, _d(ms), calls.ctypes.data_as(_lp),
                                           1 if reset else 0))
-        return {T_NAMES[i]: (ms[i], int(calls[i])) for i in range(7)}
+        return {name: (ms[i], int(calls[i])) for i, name in enumerate(T_NAMES)}
 
 
 class Species:
