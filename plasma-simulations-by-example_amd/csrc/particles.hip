@@ -473,7 +473,8 @@ constexpr int PLAN_SCAN_THREADS = 1024;
 // scratch of the plan (per species): per-tile need and slack, per-block sums, alpha
 struct PlanTmp {
     uint32_t *need, *slack;
-    uint32_t *staged;         // staged particles per current tile (zeroed again by k_plan_caps)
+    uint32_t *staged;         // staged particles per current tile, PLAN_STAGED_SHARDS partial
+                              // counts per tile (shard-major); zeroed again by k_plan_caps
     uint32_t *nchunk;         // push workgroups per tile
     unsigned long long *blk;  // [2 * nblocks]
     unsigned long long *blk2; // [nblocks] chunk block sums, then their offsets
@@ -498,7 +499,8 @@ __device__ inline void plan_sizes(const Grid &g, const PartView &in, const uint3
         if (ty < g.nty - 1) nb = max(nb, tile_count(in, t + g.ntx));
         if (tz > 0) nb = max(nb, tile_count(in, t - g.ntx * g.nty));
         if (tz < g.ntz - 1) nb = max(nb, tile_count(in, t + g.ntx * g.nty));
-        uint32_t arrivals = in.bin_in[t] + staged[t];
+        uint32_t arrivals = in.bin_in[t];
+        for (int k = 0; k < PLAN_STAGED_SHARDS; k++) arrivals += staged[(long)k * g.ntiles + t];
         s = max(max(n / 8, nb / 4), arrivals + arrivals / 2) + 64;
     } else {
         n = hist[t];
@@ -518,15 +520,18 @@ __device__ inline unsigned long long block_sum_u64(unsigned long long v, unsigne
     return r;
 }
 
-// staged particles per current tile (the staging list of the input buffer)
+// staged particles per current tile (the staging list of the input buffer).  The list is
+// mostly a step's injection, all on the inlet face: the counts are split over
+// PLAN_STAGED_SHARDS copies (by workgroup) so that fewer atomics queue on each address.
 template <int PREC>
 __global__ void __launch_bounds__(256) k_plan_staged(Grid g, PartView in, uint32_t *staged) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
     uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
+    uint32_t *mine = staged + (long)(blockIdx.x % PLAN_STAGED_SHARDS) * g.ntiles;
     for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < ns; q += gridDim.x * 256) {
         PT p;
         load_p(in, in.cap_bins + q, p);
-        atomicAdd(&staged[tile_of(g, p)], 1u);
+        atomicAdd(&mine[tile_of(g, p)], 1u);
     }
 }
 
@@ -579,7 +584,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_caps(Grid g, PlanTmp T, P
         c = own + (uint32_t)(T.alpha[0] * (double)T.slack[t]);
         out.bin_cap[t] = c;
         out.bin_off[t] = own;
-        T.staged[t] = 0;  // ready for the next plan
+        for (int k = 0; k < PLAN_STAGED_SHARDS; k++) T.staged[(long)k * g.ntiles + t] = 0;  // next plan
     }
     unsigned long long a = block_sum_u64(c, sh);
     if (threadIdx.x == 0) T.blk[blockIdx.x] = a;
@@ -731,15 +736,21 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         acc[5] += 1;
     };
 
+    // Workgroups [0, PUSH_STAGE_WGS) push the staging list, the rest the binned tiles.  The
+    // staging ones come first in dispatch order: their particles (a step's injection) gather
+    // and deposit through global memory and take longer per particle, so started last they
+    // would be the kernel's tail.
     const unsigned nbin = gridDim.x - PUSH_STAGE_WGS;  // workgroups for the binned tiles
-    if (blockIdx.x < nbin) {
+    if (blockIdx.x >= PUSH_STAGE_WGS) {
         // ------------------------------------------------ binned tile: waves run independently
         // workgroup -> (tile, chunk) from the plan's map.  Workgroups are dispatched
-        // round-robin over the 8 XCDs (XCD b % 8); the j-th one of an XCD takes entry
-        // (j / S * 8 + xcd) * S + j % S, S = PUSH_XCD_STRIPE: a bijection of [0, nbin)
-        // (nbin is a multiple of 8 S) that deals out stripes of S entries round-robin
+        // round-robin over the 8 XCDs (XCD b % 8, PUSH_STAGE_WGS a multiple of 8); the
+        // j-th binned one of an XCD takes entry (j / S * 8 + xcd) * S + j % S,
+        // S = PUSH_XCD_STRIPE: a bijection of [0, nbin) (nbin is a multiple of 8 S) that
+        // deals out stripes of S entries round-robin
         const unsigned total = (unsigned)min(*O.total_groups, (unsigned long long)nbin);
-        const unsigned xcd = blockIdx.x % 8u, j = blockIdx.x / 8u;
+        const unsigned b = blockIdx.x - PUSH_STAGE_WGS;
+        const unsigned xcd = b % 8u, j = b / 8u;
         const unsigned wg = ((j / PUSH_XCD_STRIPE) * 8u + xcd) * PUSH_XCD_STRIPE + j % PUSH_XCD_STRIPE;
         if (wg >= total) {
             if (tid < NDIAG_SLOTS) O.diag_part[tid * gridDim.x + blockIdx.x] = 0.0;
@@ -913,8 +924,8 @@ __global__ void __launch_bounds__(PUSH_THREADS)
     } else {
         // ------------------------------------------------ unbinned staging list
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
-        uint32_t nthreads = (gridDim.x - nbin) * PUSH_THREADS;
-        uint32_t first = (blockIdx.x - nbin) * PUSH_THREADS + tid;
+        uint32_t nthreads = PUSH_STAGE_WGS * PUSH_THREADS;
+        uint32_t first = blockIdx.x * PUSH_THREADS + tid;
         uint32_t trips = (ns + nthreads - 1) / nthreads;  // uniform per wavefront
         for (uint32_t it = 0; it < trips; it++) {
             uint32_t q = first + it * nthreads;
@@ -999,21 +1010,58 @@ __global__ void __launch_bounds__(256)
     den[u] = (V != 0) ? raw / V : 0.0;
 }
 
-// deposit-only pass (computeNumberDensity without a preceding push): global FP64 atomics
+// deposit-only pass (computeNumberDensity without a preceding push), on the push's windows:
+// each tile's particles go into an FP64 LDS window of the tile (HALO cells around it) that
+// is written to the tile's window slot, which k_finish_density sums; particles outside the
+// window (a bin edited from the host) and the staging list use global FP64 atomics.  The
+// caller clears win_used: empty tiles leave their slot unused.
 template <int PREC>
-__global__ void __launch_bounds__(256) k_deposit_all(Grid g, PartView in, double *den_extra) {
+__global__ void __launch_bounds__(256)
+    k_deposit_tiles(Grid g, PartView in, float *win32, double *win64, uint8_t *win_used, double *den_extra) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
-    int b = blockIdx.x;
+    __shared__ double sW[WNODES];
+    const int b = blockIdx.x, tid = threadIdx.x;
     if (b < g.ntiles) {
-        BinSeg bs = bin_of(in, b);
-        for (uint32_t q = threadIdx.x; q < bs.c0 + bs.c1; q += blockDim.x) {
+        const BinSeg bs = bin_of(in, b);
+        const uint32_t cnt = bs.c0 + bs.c1;
+        if (cnt == 0) return;
+        const int c0x = (b % g.ntx) * TILE, c0y = ((b / g.ntx) % g.nty) * TILE, c0z = (b / (g.ntx * g.nty)) * TILE;
+        for (int n = tid; n < WNODES; n += blockDim.x) sW[n] = 0.0;
+        __syncthreads();
+        for (uint32_t q = tid; q < cnt; q += blockDim.x) {
             PT p;
             load_p(in, seg_index(bs, q), p);
-            deposit_global(g, den_extra, p);
+            int ci, cj, ck;
+            double di = 0, dj = 0, dk = 0;
+            if constexpr (PREC == PIC_MIXED) {
+                ci = p.ci;
+                cj = p.cj;
+                ck = p.ck;
+            } else {
+                cell_of(g, p, ci, cj, ck, di, dj, dk);
+            }
+            const int wi = ci - c0x + HALO, wj = cj - c0y + HALO, wk = ck - c0z + HALO;
+            if (wi >= 0 && wi < TILE + 2 * HALO && wj >= 0 && wj < TILE + 2 * HALO && wk >= 0 && wk < TILE + 2 * HALO) {
+                if constexpr (PREC == PIC_MIXED)
+                    deposit_lds<float>(sW, wi, wj, wk, p.ox, p.oy, p.oz, p.w);
+                else
+                    deposit_lds<double>(sW, wi, wj, wk, di, dj, dk, p.w);
+            } else {
+                deposit_global(g, den_extra, p);
+            }
         }
+        __syncthreads();
+        if (PREC == PIC_MIXED) {
+            float *dst = win32 + (long)b * WNODES;
+            for (int n = tid; n < WNODES; n += blockDim.x) dst[n] = (float)sW[n];
+        } else {
+            double *dst = win64 + (long)b * WNODES;
+            for (int n = tid; n < WNODES; n += blockDim.x) dst[n] = sW[n];
+        }
+        if (tid == 0) win_used[b] = 1;
     } else {
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
-        for (uint32_t q = (b - g.ntiles) * blockDim.x + threadIdx.x; q < ns; q += (gridDim.x - g.ntiles) * blockDim.x) {
+        for (uint32_t q = (b - g.ntiles) * blockDim.x + tid; q < ns; q += (gridDim.x - g.ntiles) * blockDim.x) {
             PT p;
             load_p(in, in.cap_bins + q, p);
             deposit_global(g, den_extra, p);
@@ -1450,9 +1498,9 @@ static PlanTmp plan_tmp(pic_species *s) {
     size_t nblk = (nt + PLAN_THREADS - 1) / PLAN_THREADS;
     T.need = (uint32_t *)p;
     T.slack = (uint32_t *)(p + 4 * nt);
-    T.staged = (uint32_t *)(p + 8 * nt);
-    T.nchunk = (uint32_t *)(p + 12 * nt);
-    T.alpha = (double *)(p + ((16 * nt + 7) & ~(size_t)7));
+    T.nchunk = (uint32_t *)(p + 8 * nt);
+    T.staged = (uint32_t *)(p + 12 * nt);
+    T.alpha = (double *)(p + (((12 + 4 * (size_t)PLAN_STAGED_SHARDS) * nt + 7) & ~(size_t)7));
     T.total = (unsigned long long *)((char *)T.alpha + 16);
     T.blk = T.total + 1;
     T.blk2 = T.blk + 2 * nblk;
@@ -1461,7 +1509,7 @@ static PlanTmp plan_tmp(pic_species *s) {
 
 static size_t plan_bytes(const Grid &g) {
     size_t nblk = ((size_t)g.ntiles + PLAN_THREADS - 1) / PLAN_THREADS;
-    return ((16 * (size_t)g.ntiles + 7) & ~(size_t)7) + 16 + 8 + 24 * nblk;
+    return (((12 + 4 * (size_t)PLAN_STAGED_SHARDS) * (size_t)g.ntiles + 7) & ~(size_t)7) + 16 + 8 + 24 * nblk;
 }
 
 // push workgroups over the tiles: at most one per tile plus one per PUSH_CHUNK bin slots,
@@ -1502,9 +1550,9 @@ static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist,
     PartView vo = view_of(out);
     if (mode == 0) {
         if (s->prec == PIC_MIXED)
-            k_plan_staged<PIC_MIXED><<<STAGE_WGS, 256, 0, w->stream>>>(g, in, T.staged);
+            k_plan_staged<PIC_MIXED><<<PLAN_STAGED_WGS, 256, 0, w->stream>>>(g, in, T.staged);
         else
-            k_plan_staged<PIC_FP64><<<STAGE_WGS, 256, 0, w->stream>>>(g, in, T.staged);
+            k_plan_staged<PIC_FP64><<<PLAN_STAGED_WGS, 256, 0, w->stream>>>(g, in, T.staged);
     }
     k_plan_count<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, in, hist, mode, T);
     k_plan_total<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T, nblk, (unsigned long long)out.cap_bins, err);
@@ -1968,9 +2016,11 @@ int pic_species_compute_number_density(pic_species *s) {
         unsigned grid = g.ntiles + STAGE_WGS;
         PartView v = view_of(s->buf[s->cur]);
         if (s->prec == PIC_MIXED)
-            k_deposit_all<PIC_MIXED><<<grid, 256, 0, w->stream>>>(g, v, s->den_extra);
+            k_deposit_tiles<PIC_MIXED><<<grid, 256, 0, w->stream>>>(g, v, (float *)s->win, (double *)s->win,
+                                                                    s->win_used, s->den_extra);
         else
-            k_deposit_all<PIC_FP64><<<grid, 256, 0, w->stream>>>(g, v, s->den_extra);
+            k_deposit_tiles<PIC_FP64><<<grid, 256, 0, w->stream>>>(g, v, (float *)s->win, (double *)s->win,
+                                                                   s->win_used, s->den_extra);
         PIC_LAUNCH_CHECK();
     }
     if (s->prec == PIC_MIXED)

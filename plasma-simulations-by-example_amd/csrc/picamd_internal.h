@@ -41,11 +41,18 @@ constexpr int STAGE_WGS = 64;             // workgroups that walk the unbinned l
 // rest.  PUSH_STAGE_WGS workgroups push the unbinned list.
 constexpr int PUSH_CHUNK = 32768;
 constexpr int PUSH_STAGE_WGS = 256;
+// The staged histogram of the push plan walks the staging list, which holds a step's
+// injection (1.75e6 particles per step in the sphere configurations): a wide grid keeps its
+// per-thread chain of loads short, and PLAN_STAGED_SHARDS partial counts per tile spread the
+// atomics of particles that share a tile.
+constexpr int PLAN_STAGED_WGS = 1024;
+constexpr int PLAN_STAGED_SHARDS = 16;
 // Push workgroups are handed to the 8 XCDs in stripes of PUSH_XCD_STRIPE consecutive
 // (tile, chunk) entries, round-robin: each XCD's L2 serves runs of neighbouring tiles, and
 // every XCD gets a share of each region of the domain.  (Contiguous eighths per XCD left
 // the XCDs that own the wake running at twice the others' time.)
 constexpr int PUSH_XCD_STRIPE = 32;
+static_assert(PUSH_STAGE_WGS % 8 == 0, "binned push workgroups keep their XCD (blockIdx % 8)");
 
 // packed cell index of PIC_MIXED particles: i | j<<10 | k<<20
 constexpr int CELL_BITS_I = 10, CELL_BITS_J = 10, CELL_BITS_K = 11;

@@ -94,10 +94,45 @@ def test_device_box_load_and_sort_invariants():
     p2, v2, m2, i2 = sp.particles()
     o1, o2 = np.argsort(ids), np.argsort(i2)
     assert np.array_equal(pos[o1], p2[o2]) and np.array_equal(vel[o1], v2[o2])
-    # density integrates to the loaded weight
+    # density integrates to the loaded weight (the mixed form's tile windows are FP32)
     sp.compute_number_density()
     nv = g.node_volumes()
-    assert abs(np.sum(sp.den() * nv) / mpw.sum() - 1) < 1e-12
+    assert abs(np.sum(sp.den() * nv) / mpw.sum() - 1) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [P.MIXED, P.FP64])
+def test_deposit_only_tile_windows_and_staging_match_oracle(prec):
+    """computeNumberDensity without a push (ch2/Species.cpp:91-97): particles in tile bins
+    deposit through the tiles' LDS windows, the staging list through global atomics; both
+    against the oracle's scatter of the particles the device holds."""
+    n = (33, 33, 65)
+    w, g = sphere_world(n)
+    sp = P.Species(w, 16 * O.AMU, O.QE, 1.0, prec, capacity=300000)
+    rng = np.random.default_rng(9)
+    nv = g.node_volumes()
+    tol = 1e-12 if prec == P.FP64 else 2e-6  # FP32 windows in the mixed form
+
+    def check():
+        sp.compute_number_density()
+        pos, _, mpw, _ = sp.particles()
+        ref = O.number_density(g, pos, mpw, nv)
+        np.testing.assert_allclose(sp.den(), ref, rtol=tol, atol=tol * np.abs(ref).max())
+
+    # the last node plane is reachable only from cells of the last tile row: cover the edges
+    pos = g.x0 + rng.random((150000, 3)) * (g.xm - g.x0)
+    pos[:2000] = g.xm - rng.random((2000, 3)) * 1e-9 * (g.xm - g.x0)
+    mpw = rng.uniform(0.5, 2.0, len(pos))
+    sp.set_particles(pos, rng.normal(0, 100, pos.shape), mpw)
+    check()            # staging list only
+    sp.sort()
+    sp.set_particles(pos, rng.normal(0, 100, pos.shape), mpw)  # re-set: density invalid again
+    sp.sort()
+    check()            # tile bins only
+    more = g.x0 + rng.random((40000, 3)) * (g.xm - g.x0)
+    sp.add_particles(more, np.zeros_like(more), np.ones(len(more)), 1e-7)  # E = 0: no rewind
+    check()            # bins and staging list together
+    nvol = np.sum(sp.den() * nv)
+    assert abs(nvol / (mpw.sum() + len(more)) - 1) < (1e-12 if prec == P.FP64 else 1e-6)
 
 
 def test_update_averages_running_mean():
