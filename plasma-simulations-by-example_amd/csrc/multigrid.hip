@@ -49,6 +49,12 @@ struct MgLevel {
 
 struct MgDev;
 
+#ifndef MG_NU
+#define MG_NU 1
+#endif
+// red-black sweep pairs per side of the V-cycle on the levels outside the fused coarse cycle
+constexpr int MG_SWEEPS = MG_NU;
+
 struct MgHierarchy {
     std::vector<MgLevel> L;
     int coarse_sweeps = 0;
@@ -582,6 +588,10 @@ static void down(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevSc
     MgLevel &f = H->L[l], &c = H->L[l + 1];
     Coef6 C = coef6(f.Q);
     k_mg_pre<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, sc);
+    for (int it = 1; it < MG_SWEEPS; it++) {  // further (red, black) pairs
+        k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
+        k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
+    }
     k_mg_residual<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, f.r, sc);
     k_mg_restrict<<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.r, c.r, sc);
     k_mg_fold<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
@@ -605,6 +615,10 @@ static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScal
     k_mg_extend<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
     k_mg_prolong<<<smooth_blocks(f.g), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
     k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
+    for (int it = 1; it < MG_SWEEPS; it++) {  // the pre-smoothing pairs in reverse: (black, red)
+        k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
+        k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
+    }
     if (l > 0) k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
 }
 

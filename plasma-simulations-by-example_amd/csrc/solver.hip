@@ -1215,6 +1215,10 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     int pcg_its = 0;
     for (int it = 0; it < NR_MAX_IT; it++) {
         k_nrs_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz);
+        // Each correction solve starts from y = 0, whose residual is the (small) Newton
+        // residual F, where the reference's warm start from the previous correction (kept
+        // by the PCG path) begins about as far from the new correction as that one is.
+        if (it > 0) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
         PIC_LAUNCH_CHECK();
         bool pc = false;
         if (use_mg) {
