@@ -393,7 +393,10 @@ struct MgDev {
     float *x, *r, *b;
 };
 
-constexpr long MG_FUSE_NODES = 12000;  // about 12 nodes per thread and step
+#ifndef MG_FUSE_MAX
+#define MG_FUSE_MAX 12000
+#endif
+constexpr long MG_FUSE_NODES = MG_FUSE_MAX;  // about 12 nodes per thread and step
 constexpr int MG_FUSE_THREADS = 1024;
 
 __global__ void __launch_bounds__(MG_FUSE_THREADS)
@@ -623,28 +626,29 @@ static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScal
 }
 
 // x0 = V(b0): one V-cycle from a zero initial guess on every level; x0 is level 0's
-// solution (FP32), b0 the CG residual (FP64).  With dot_partials, the last half-sweep also
-// writes per-workgroup partials of b0.x0 (mg_dot_partials of them).
-int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, double *dot_partials, hipStream_t st,
+// solution (FP32), b0 the CG residual (FP64) and b0f its FP32 copy.  The sweeps of level 0
+// read b0f (half the bytes, and the same value: they compute in FP32); the last half-sweep
+// reads b0, for its FP64 partials of b0.x0 (mg_dot_partials of them) when asked for.
+int mg_vcycle(MgHierarchy *H, const double *b0, const float *b0f, float **x0, double *dot_partials, hipStream_t st,
               const DevScalars *sc) {
     const int nl = (int)H->L.size();
     *x0 = H->L[0].x;
     MgLevel &f0 = H->L[0];
     if (nl == 1) {  // no coarsening possible: the preconditioner is one symmetric sweep pair
         Coef6 C = coef6(f0.Q);
-        k_mg_pre<double><<<nbk8(f0.g.nn), 256, 0, st>>>(f0.g, C, f0.mask, f0.idiag, f0.x, b0, sc);
-        k_mg_smooth<double><<<smooth_blocks(f0.g), 256, 0, st>>>(f0.g, C, f0.mask, f0.idiag, f0.x, b0, 1, sc);
+        k_mg_pre<float><<<nbk8(f0.g.nn), 256, 0, st>>>(f0.g, C, f0.mask, f0.idiag, f0.x, b0f, sc);
+        k_mg_smooth<float><<<smooth_blocks(f0.g), 256, 0, st>>>(f0.g, C, f0.mask, f0.idiag, f0.x, b0f, 1, sc);
         last_sweep(f0, b0, dot_partials, st, sc);
         PIC_LAUNCH_CHECK();
         return PIC_OK;
     }
-    down<double>(H, 0, b0, st, sc);
+    down<float>(H, 0, b0f, st, sc);
     if (H->fuse_l0 > 0) {
         const int l0 = H->fuse_l0;
         for (int l = 1; l < l0; l++) down<float>(H, l, H->L[l].b, st, sc);
         k_mg_coarse<<<1, MG_FUSE_THREADS, 0, st>>>(H->dev, l0, nl, H->coarse_sweeps, sc);
         for (int l = l0 - 1; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);
-        up<double>(H, 0, b0, st, sc);
+        up<float>(H, 0, b0f, st, sc);
         last_sweep(f0, b0, dot_partials, st, sc);
         PIC_LAUNCH_CHECK();
         return PIC_OK;
@@ -665,7 +669,7 @@ int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, double *dot_partials
         }
     }
     for (int l = nl - 2; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);
-    up<double>(H, 0, b0, st, sc);
+    up<float>(H, 0, b0f, st, sc);
     last_sweep(f0, b0, dot_partials, st, sc);
     PIC_LAUNCH_CHECK();
     return PIC_OK;

@@ -221,6 +221,7 @@ struct pic_solver {
     uint8_t *mask = nullptr;  // PCG_SYM: regular-neighbour bits per node
     // PIC_SOLVER_MG: Boltzmann derivative of the Newton step, multigrid levels
     double *boltz = nullptr;
+    float *g32 = nullptr;  // FP32 copy of the CG residual, the V-cycle's right-hand side
     pic::MgHierarchy *mg = nullptr;
 };
 

@@ -667,11 +667,13 @@ __global__ void __launch_bounds__(256)
 // Scalars: red[0] a, red[1] b, red[4] RMS(g), red[6] mu, red[7] lam = a/b.
 __global__ void __launch_bounds__(256)
     k_mgpcg_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
-                 double *d0) {
+                 float *g32, double *d0) {
     long u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= Q.g.nn) return;
     uint8_t m = mask[u];
-    gv[u] = (m & 64) ? jr_apply(Q, u, m, jr[u], y) - F[u] : 0.0;
+    double gg = (m & 64) ? jr_apply(Q, u, m, jr[u], y) - F[u] : 0.0;
+    gv[u] = gg;
+    g32[u] = (float)gg;
     d0[u] = 0;
 }
 
@@ -723,7 +725,7 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void __launch_bounds__(256)
-    k_mgpcg_b(long nn, double *gv, const double *z, double *partials, const DevScalars *sc) {
+    k_mgpcg_b(long nn, double *gv, float *g32, const double *z, double *partials, const DevScalars *sc) {
     __shared__ double sh[4];
     if (sc->done) return;
     NodeChunk ch = pcg_chunk(nn);
@@ -732,6 +734,7 @@ __global__ void __launch_bounds__(256)
     for (long u = ch.u0 + threadIdx.x; u < ch.u1; u += 256) {
         double gg = gv[u] + lam * z[u];
         gv[u] = gg;
+        g32[u] = (float)gg;
         g2 += gg * gg;
     }
     double r = block_sum(g2, sh);
@@ -1121,7 +1124,7 @@ static int gsls_linear(pic_solver *s, const PcgParams &Q) {
 // Newton iteration of solveNRPCG (:225-296) on the Neumann-reduced system
 int mg_build(pic_solver *s, MgHierarchy **out);
 int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, hipStream_t st);
-int mg_vcycle(MgHierarchy *H, const double *b0, float **x0, double *dot_partials, hipStream_t st,
+int mg_vcycle(MgHierarchy *H, const double *b0, const float *b0f, float **x0, double *dot_partials, hipStream_t st,
               const DevScalars *sc);
 int mg_dot_partials(const MgHierarchy *H);
 void mg_destroy(MgHierarchy *H);
@@ -1140,11 +1143,11 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     double *dbuf[2] = {s->d, s->z2};
     hipStream_t st = w->stream;
     k_reset_flags<<<1, 1, 0, st>>>(w->scal);
-    k_mgpcg_init<<<nb8, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, dbuf[0]);
+    k_mgpcg_init<<<nb8, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, s->g32, dbuf[0]);
     // s = V(g) in FP32; the V-cycle's last sweep also writes the partials of g.s
     float *sv = nullptr;
     const long ndot = mg_dot_partials(s->mg);
-    rc = mg_vcycle(s->mg, s->g, &sv, w->partials, st, w->scal);
+    rc = mg_vcycle(s->mg, s->g, s->g32, &sv, w->partials, st, w->scal);
     if (rc) return rc;
     k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, ndot, folded, w->scal);
     k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 0, nu, Q.tol, 0, w->scal);
@@ -1159,9 +1162,9 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
                                          w->partials, w->scal);
             k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, nb8, folded, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
-            k_mgpcg_b<<<G, 256, 0, st>>>(nn, s->g, s->z, w->partials, w->scal);
+            k_mgpcg_b<<<G, 256, 0, st>>>(nn, s->g, s->g32, s->z, w->partials, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(w->partials, (int)G, 2, nu, Q.tol, it, w->scal);
-            rc = mg_vcycle(s->mg, s->g, &sv, w->partials, st, w->scal);
+            rc = mg_vcycle(s->mg, s->g, s->g32, &sv, w->partials, st, w->scal);
             if (rc) return rc;
             k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, ndot, folded, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 3, nu, Q.tol, it, w->scal);
@@ -1184,6 +1187,7 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
 
 static int ensure_mg(pic_solver *s) {
     if (!s->boltz) PIC_HIP(hipMalloc(&s->boltz, (size_t)s->w->g.nn * sizeof(double)));
+    if (!s->g32) PIC_HIP(hipMalloc(&s->g32, (size_t)s->w->g.nn * sizeof(float)));
     return mg_build(s, &s->mg);
 }
 
@@ -1288,6 +1292,7 @@ int pic_solver_destroy(pic_solver *s) {
     hipFree(s->mask);
     mg_destroy(s->mg);
     hipFree(s->boltz);
+    hipFree(s->g32);
     delete s;
     return PIC_OK;
 }
