@@ -274,17 +274,20 @@ __device__ inline void mg_prolong_node(const Grid &gf, const Grid &gc, const uin
     int i = (int)(2 * ih) + ((j + k) & 1);
     if (i >= gf.ni) return;
     long u = i + (long)gf.ni * row;
-    if (!(mask_f[u] & 64)) return;
+    // the mask, the coarse values and x are loaded together (an odd fine index has its
+    // upper coarse neighbour inside the coarse grid), and the mask decides the store
     int I = i >> 1, J = j >> 1, K = k >> 1;
     int ai = i & 1, aj = j & 1, ak = k & 1;
     long sj = gc.ni, sk = (long)gc.ni * gc.nj;
     long b = I + sj * J + sk * K;
+    uint8_t m = mask_f[u];
+    float xu = xf[u];
     float s = 0;
     for (int c = 0; c <= ak; c++)
         for (int bb = 0; bb <= aj; bb++)
             for (int a = 0; a <= ai; a++) s += xc[b + a + sj * bb + sk * c];
     // weights: 1, 1/2, 1/4 or 1/8 for 1, 2, 4 or 8 coarse nodes
-    xf[u] += s * (1.0f / (float)(1 << (ai + aj + ak)));
+    if (m & 64) xf[u] = xu + s * (1.0f / (float)(1 << (ai + aj + ak)));
 }
 
 // ---- per-level kernels (one node per thread, XCD-remapped workgroups) ---------------------
