@@ -29,7 +29,8 @@
 extern "C" {
 #endif
 
-#define PICAMD_ABI_VERSION 1
+/* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL) */
+#define PICAMD_ABI_VERSION 2
 
 /* status codes */
 #define PIC_OK 0
