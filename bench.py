@@ -61,7 +61,8 @@ def parse():
                          "field and history comparison")
     ap.add_argument("--push-only", action="store_true",
                     help="profiling mode: the same particle workload, pushes only (no solve), "
-                         "plus a 1 GiB device copy that calibrates the HBM byte counters")
+                         "then one Field_::updateAverage pass (a plain streaming kernel of known "
+                         "bytes) that calibrates the HBM byte counters")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
                     help="particle form of --push-only")
     return ap.parse_args()

@@ -563,12 +563,24 @@ __global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid
 // F on regular nodes (the reference's residual, ch3/ver2/PotentialSolver.cpp:240-257), the
 // reduced diagonal jr = A(u,u) - P(u) + sum of Neumann-neighbour coefficients, and a mask:
 // bits 0-5 regular neighbours in direction d, bit 6 the node itself is regular.
+// CgStart (nullable): the start of a correction solve from y = 0, written here instead of
+// by a pass of its own: g = J 0 - F = -F (FP64 and FP32), d = 0, y = 0
+struct CgStart {
+    double *g;
+    float *g32;
+    double *d0, *y;
+};
+
 __global__ void __launch_bounds__(256)
     k_nrs_F(PcgParams Q, const double *x, const double *rho, const uint8_t *oid, double *F, double *jr,
-            uint8_t *mask, double *boltz) {
+            uint8_t *mask, double *boltz, CgStart st) {
     long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
+    if (st.g) {
+        st.d0[u] = 0;
+        st.y[u] = 0;
+    }
     int i, j, k;
     node_ijk(g, u, i, j, k);
     int t = node_type(g, oid, u, i, j, k);
@@ -577,6 +589,10 @@ __global__ void __launch_bounds__(256)
         jr[u] = 1;
         mask[u] = 0;
         if (boltz) boltz[u] = 0;
+        if (st.g) {
+            st.g[u] = 0;
+            st.g32[u] = 0.0f;
+        }
         return;
     }
     double ad = a_diag(Q, t, i, j);
@@ -597,6 +613,10 @@ __global__ void __launch_bounds__(256)
     jr[u] = diag;
     mask[u] = m;
     if (boltz) boltz[u] = p;
+    if (st.g) {
+        st.g[u] = -f;
+        st.g32[u] = (float)(-f);
+    }
 }
 
 
@@ -660,22 +680,12 @@ __global__ void __launch_bounds__(256)
 
 // ---- CG preconditioned by one multigrid V-cycle (PIC_SOLVER_MG) -------------------------
 // The iteration of solvePCGLinear (:299-331) with s = V(g) in place of s = M g:
-//   init:    g = J y - F;  s = V(g);  a = g.s;  d_0 = 0
+//   init:    g = J y - F = -F (y = 0; k_nrs_F writes g, its FP32 copy, y = 0 and d_0 = 0);
+//            s = V(g);  a = g.s
 //   pass A:  d = mu d_old - s (at the 7 stencil points), y += lam d_old, z = J d, partials d.z
 //   pass B:  g += (a/b) z, partials g.g (convergence: RMS(g) < tol)
 //   s = V(g), with the partials of g.s from its last sweep:  mu = a'/a
 // Scalars: red[0] a, red[1] b, red[4] RMS(g), red[6] mu, red[7] lam = a/b.
-__global__ void __launch_bounds__(256)
-    k_mgpcg_init(PcgParams Q, const uint8_t *mask, const double *jr, const double *y, const double *F, double *gv,
-                 float *g32, double *d0) {
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= Q.g.nn) return;
-    uint8_t m = mask[u];
-    double gg = (m & 64) ? jr_apply(Q, u, m, jr[u], y) - F[u] : 0.0;
-    gv[u] = gg;
-    g32[u] = (float)gg;
-    d0[u] = 0;
-}
 
 __global__ void __launch_bounds__(256)
     k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const float *sv, const double *d_old,
@@ -1143,7 +1153,6 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     double *dbuf[2] = {s->d, s->z2};
     hipStream_t st = w->stream;
     k_reset_flags<<<1, 1, 0, st>>>(w->scal);
-    k_mgpcg_init<<<nb8, 256, 0, st>>>(Q, s->mask, s->P, s->y, s->F, s->g, s->g32, dbuf[0]);
     // s = V(g) in FP32; the V-cycle's last sweep also writes the partials of g.s
     float *sv = nullptr;
     const long ndot = mg_dot_partials(s->mg);
@@ -1211,18 +1220,21 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     bool verbose = venv && venv[0] == '1';
     auto t0 = std::chrono::steady_clock::now();
     k_neumann_copy<<<nb8, 256, 0, w->stream>>>(w->g, w->object_id, w->phi);
-    k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
+    if (!use_mg) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
     const int NR_MAX_IT = 20;
     const double NR_TOL = 1e-3;
     bool converged = false;
     double norm = 0;
     int pcg_its = 0;
     for (int it = 0; it < NR_MAX_IT; it++) {
-        k_nrs_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz);
         // Each correction solve starts from y = 0, whose residual is the (small) Newton
         // residual F, where the reference's warm start from the previous correction (kept
         // by the PCG path) begins about as far from the new correction as that one is.
-        if (it > 0) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
+        // With the multigrid preconditioner, k_nrs_F also writes that start.
+        CgStart cs{nullptr, nullptr, nullptr, nullptr};
+        if (use_mg) cs = CgStart{s->g, s->g32, s->d, s->y};
+        k_nrs_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz, cs);
+        if (!use_mg && it > 0) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
         PIC_LAUNCH_CHECK();
         bool pc = false;
         if (use_mg) {
