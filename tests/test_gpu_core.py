@@ -102,14 +102,14 @@ def test_solve_qn_matches_closed_form():
 
 
 def _push_case(prec, bc, steps, dt, vmax, sphere=None, n=20000, grid=(21, 21, 21), x0=C1_X0,
-               xm=C1_XM, seed=11, amp=200.0, charge=-O.QE, mass=O.ME):
+               xm=C1_XM, seed=11, amp=200.0, charge=-O.QE, mass=O.ME, box=None):
     w, g = make_world(grid, x0, xm)
     phi, ef = smooth_field(g, seed, amp)
     if sphere is not None:
         w.add_sphere(*sphere)
     w.set_field(P.EF, ef)
     sp = P.Species(w, mass, charge, 1e4, prec, capacity=n, track_ids=True)
-    pos, vel, mpw = random_particles(g, n, seed + 1, vmax)
+    pos, vel, mpw = random_particles(g, n, seed + 1, vmax, box)
     if sphere is not None:  # start outside the sphere
         keep = np.sum((pos - np.asarray(sphere[0])) ** 2, axis=1) > sphere[1] ** 2 * 1.02
         pos, vel, mpw = pos[keep], vel[keep], mpw[keep]
@@ -169,6 +169,32 @@ def test_push_reflect_mixed_matches_oracle(vmax, dt):
     mom_scale = O.ME * np.sum(r["rw"][:, None] * np.abs(r["rv"]))
     np.testing.assert_allclose(r["diag"][[0, 4]], d[[0, 4]], rtol=1e-5)
     np.testing.assert_allclose(r["diag"][1:4], d[1:4], rtol=0, atol=1e-5 * mom_scale)
+
+
+# 17^3 nodes = 2x2x2 tiles of 8^3 cells; 200,000 particles in the first tile, where one push
+# workgroup takes 32,768 of them: seven workgroups share the tile (stayer slots by global
+# atomics, later chunks' windows added to the far-deposit array), and movers cross into
+# the neighbouring tiles
+SPLIT = dict(n=200_000, grid=(17, 17, 17), box=((-0.1, -0.1, -0.1), (-0.004, -0.004, -0.004)))
+
+
+def test_push_split_tile_fp64_matches_oracle():
+    r = _push_case(P.FP64, P.REFLECT, 6, 1e-9, 3e6, **SPLIT)
+    assert len(r["gi"]) == SPLIT["n"] and np.array_equal(r["gi"], r["ri"])
+    np.testing.assert_allclose(r["gp"], r["rp"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(r["gv"], r["rv"], rtol=1e-14, atol=1e-9)
+    assert rel_l2(r["den"], r["den_ref"]) < 1e-13
+    # the particles left the first tile's cells
+    assert np.any(r["rp"] > -0.0)
+
+
+def test_push_split_tile_mixed_matches_oracle():
+    r = _push_case(P.MIXED, P.REFLECT, 6, 1e-9, 3e6, **SPLIT)
+    g = r["g"]
+    assert len(r["gi"]) == SPLIT["n"] and np.array_equal(r["gi"], r["ri"])
+    dcell = np.abs(r["gp"] - r["rp"]) / g.dh
+    assert dcell.max() < 2e-4, dcell.max()
+    assert rel_l2(r["den"], r["den_ref"]) < 1e-5
 
 
 def test_push_absorb_sphere_fp64_matches_oracle():
