@@ -985,9 +985,8 @@ __global__ void __launch_bounds__(256)
 #pragma clang fp contract(off)
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
-    int i = (int)(u % g.ni);
-    int j = (int)((u / g.ni) % g.nj);
-    int k = (int)(u / ((long)g.ni * g.nj));
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
     double raw = den_extra[u];
     den_extra[u] = 0;  // ready for the next deposit
     // tiles whose window [a*TILE-HALO, a*TILE+TILE+HALO] holds node i
@@ -2024,10 +2023,10 @@ int pic_species_compute_number_density(pic_species *s) {
         PIC_LAUNCH_CHECK();
     }
     if (s->prec == PIC_MIXED)
-        k_finish_density<float><<<nb256(g.nn), 256, 0, w->stream>>>(g, (const float *)s->win, s->win_used,
+        k_finish_density<float><<<nbk8(g.nn), 256, 0, w->stream>>>(g, (const float *)s->win, s->win_used,
                                                                     s->den_extra, s->den);
     else
-        k_finish_density<double><<<nb256(g.nn), 256, 0, w->stream>>>(g, (const double *)s->win, s->win_used,
+        k_finish_density<double><<<nbk8(g.nn), 256, 0, w->stream>>>(g, (const double *)s->win, s->win_used,
                                                                      s->den_extra, s->den);
     PIC_LAUNCH_CHECK();
     timer_end(w, PIC_T_DEPOSIT, ev);

@@ -8,27 +8,6 @@
 
 namespace pic {
 
-// (i, j, k) of flat node u = k*ni*nj + j*ni + i; 32-bit unsigned arithmetic (grids hold
-// fewer than 2^32 nodes), which divides in a few instructions where 64-bit division takes
-// dozens
-__device__ inline void node_ijk(const Grid &g, long u, int &i, int &j, int &k) {
-    unsigned uu = (unsigned)u, ni = (unsigned)g.ni, nj = (unsigned)g.nj;
-    unsigned r = uu / ni;
-    i = (int)(uu - r * ni);
-    unsigned kk = r / nj;
-    j = (int)(r - kk * nj);
-    k = (int)kk;
-}
-
-// Workgroup index for node-stencil kernels.  Workgroups are dispatched round-robin over the
-// 8 XCDs, each with its own L2; with this remap XCD x runs workgroups [x G/8, (x+1) G/8),
-// a contiguous slab of k planes, so the +-j and +-k neighbours a workgroup reads were
-// fetched into the same L2 by the workgroups next to it.  Launch with nbk8() workgroups.
-__device__ inline long xcd_block() {
-    unsigned G = gridDim.x, b = blockIdx.x;
-    return (G % 8) ? (long)b : (long)((b % 8) * (G / 8) + b / 8);
-}
-
 // Node classes: regular (an unknown), Neumann face (copies the node its chain reaches),
 // Dirichlet (object_id > 0: sphere, inlet).
 enum { NT_REG = 0, NT_NEU = 1, NT_DIR = 2 };
@@ -95,8 +74,6 @@ __device__ inline double fixed_sum(const double *p, int nb, double *sh) {
     return block_sum(s, sh);
 }
 
-inline unsigned nbk(long n) { return (unsigned)((n + 255) / 256); }
-inline unsigned nbk8(long n) { return (nbk(n) + 7) / 8 * 8; }
 
 }  // namespace pic
 #endif

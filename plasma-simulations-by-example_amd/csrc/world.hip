@@ -77,9 +77,8 @@ static void resolve_timers(pic_world *w) {
 __global__ void k_node_volumes(Grid g, double *nv) {
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
-    int i = (int)(u % g.ni);
-    int j = (int)((u / g.ni) % g.nj);
-    int k = (int)(u / ((long)g.ni * g.nj));
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
     // World::computeNodeVolumes (ch2/World.cpp:58-69)
     double V = g.dh[0] * g.dh[1] * g.dh[2];
     if (i == 0 || i == g.ni - 1) V *= 0.5;
@@ -92,9 +91,8 @@ __global__ void k_add_sphere(Grid g, uint8_t *oid, double *phi, double cx, doubl
                              double r2, double phi_s) {
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (u >= g.nn) return;
-    int i = (int)(u % g.ni);
-    int j = (int)((u / g.ni) % g.nj);
-    int k = (int)(u / ((long)g.ni * g.nj));
+    int i, j, k;
+    node_ijk(g, u, i, j, k);
     // World::pos(i,j,k) = x0 + dh*(i,j,k); World::inSphere (ch3/ver2/World.cpp:88-126)
     double x = g.x0[0] + g.dh[0] * (double)i;
     double y = g.x0[1] + g.dh[1] * (double)j;
@@ -126,9 +124,8 @@ __global__ void __launch_bounds__(256) k_compute_ef(Grid g, const double *__rest
     double pe = 0;
     if (u < g.nn) {
         int ni = g.ni, nj = g.nj, nk = g.nk;
-        int i = (int)(u % ni);
-        int j = (int)((u / ni) % nj);
-        int k = (int)(u / ((long)ni * nj));
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
         long si = 1, sj = ni, sk = (long)ni * nj;
         double dx = g.dh[0], dy = g.dh[1], dz = g.dh[2];
         double ex, ey, ez;
@@ -168,9 +165,8 @@ __global__ void __launch_bounds__(256) k_pe_from_ef(Grid g, const double4 *ef64,
     long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double pe = 0;
     if (u < g.nn) {
-        int i = (int)(u % g.ni);
-        int j = (int)((u / g.ni) % g.nj);
-        int k = (int)(u / ((long)g.ni * g.nj));
+        int i, j, k;
+        node_ijk(g, u, i, j, k);
         double V = g.dh[0] * g.dh[1] * g.dh[2];
         if (i == 0 || i == g.ni - 1) V *= 0.5;
         if (j == 0 || j == g.nj - 1) V *= 0.5;
@@ -247,7 +243,7 @@ int compute_ef(pic_world *w, bool want32, bool want64) {
               "computeEF needs at least 3 nodes per axis");
     hipEvent_t ev;
     timer_begin(w, PIC_T_EF, &ev);
-    unsigned nb = nblocks(g.nn, 256);
+    unsigned nb = nbk8(g.nn);  // one PE partial per workgroup (nbk8: the partials' count)
     if (want32 && want64)
         k_compute_ef<true, true><<<nb, 256, 0, w->stream>>>(g, w->phi, w->ef32, w->ef64, w->pe_part);
     else if (want32)
@@ -331,8 +327,8 @@ int pic_world_create(int ni, int nj, int nk, pic_world **out) {
     TRY(hipMalloc(&w->ef32, sizeof(float4) * g.nn));
     TRY(hipMalloc(&w->ef64, sizeof(double4) * g.nn));
     TRY(hipMalloc(&w->object_id, g.nn));
-    TRY(hipMalloc(&w->pe_part, sizeof(double) * ((g.nn + 255) / 256)));
-    TRY(hipMemsetAsync(w->pe_part, 0, sizeof(double) * ((g.nn + 255) / 256), w->stream));
+    TRY(hipMalloc(&w->pe_part, sizeof(double) * nbk8(g.nn)));
+    TRY(hipMemsetAsync(w->pe_part, 0, sizeof(double) * nbk8(g.nn), w->stream));
     TRY(hipMalloc(&w->scal, sizeof(DevScalars)));
     TRY(hipHostMalloc(&w->scal_host, sizeof(DevScalars)));
     TRY(hipMemsetAsync(w->phi, 0, sizeof(double) * g.nn, w->stream));
@@ -499,11 +495,11 @@ int pic_world_get_pe(pic_world *w, double *pe) {
     PIC_CHECK(w && pe, PIC_ERR_ARG, "null argument");
     if (!w->pe_valid) {
         // E was written from the host: sum |E|^2 V over the stored field
-        k_pe_from_ef<<<nblocks(w->g.nn, 256), 256, 0, w->stream>>>(w->g, w->ef64, w->pe_part);
+        k_pe_from_ef<<<nbk8(w->g.nn), 256, 0, w->stream>>>(w->g, w->ef64, w->pe_part);
         PIC_LAUNCH_CHECK();
         w->pe_valid = true;
     }
-    k_sum_partials<<<1, 1024, 0, w->stream>>>(w->pe_part, (long)nblocks(w->g.nn, 256), &w->scal->pe);
+    k_sum_partials<<<1, 1024, 0, w->stream>>>(w->pe_part, (long)nbk8(w->g.nn), &w->scal->pe);
     PIC_LAUNCH_CHECK();
     PIC_HIP(hipMemcpyAsync(&w->scal_host->pe, &w->scal->pe, sizeof(double), hipMemcpyDeviceToHost,
                            w->stream));
