@@ -11,9 +11,11 @@
 //
 // Hierarchy.  Vertex-centred coarsening by 2 in every direction while every (n-1) is even
 // and every n >= 5 (257x257x513 -> 129x129x257 -> ... -> 3x3x5).  Coarse nodes take their
-// object id and Boltzmann derivative P from the fine node at the same position (injection);
-// each level discretises the reduced 7-point operator at its own spacing, with Neumann face
-// nodes eliminated into the diagonal exactly as on the finest level.
+// object id and Boltzmann derivative P from the fine node at the same position (injection).
+// Level 1 discretises the reduced 7-point operator at its own spacing, with Neumann face
+// nodes eliminated into the diagonal exactly as on the finest level; levels from
+// MG_GALERKIN_FROM on carry the Galerkin product R A P of the level above (27 points, built
+// once per geometry) with the injected P on its diagonal.
 //
 // Transfers.  Prolongation is trilinear interpolation of the coarse correction after its
 // face nodes are filled from their copy-chain targets (E), written to regular fine nodes
@@ -21,8 +23,9 @@
 // into their chain targets, E^T), so that the V-cycle is a symmetric operator.
 //
 // Smoothing.  One red-black Gauss-Seidel sweep before (red, black) and one after (black,
-// red) the coarse correction: the post-smoother is the adjoint of the pre-smoother, as CG
-// requires of its preconditioner.  The coarsest level is solved by sweeps.
+// red) the coarse correction on the 7-point levels, two damped-Jacobi sweeps on each side on
+// the Galerkin levels: the post-smoother is the adjoint of the pre-smoother, as CG requires
+// of its preconditioner.  The coarsest level is solved by sweeps.
 //
 // Every kernel here returns at once when the solve has converged (DevScalars::done), so
 // the host can queue several iterations between convergence checks.
@@ -44,6 +47,9 @@ struct MgLevel {
     float *boltz = nullptr;  // Boltzmann derivative P (levels >= 1)
     float *x = nullptr, *r = nullptr;
     float *b = nullptr;      // right-hand side (levels >= 1; level 0 reads the CG residual)
+    float *a27 = nullptr;    // Galerkin levels: the 27 coefficients of R J P (plane-major)
+    float *x2 = nullptr;     // Galerkin levels: the second Jacobi buffer
+    bool gal = false;
     bool owned = false;
 };
 
@@ -55,8 +61,26 @@ struct MgDev;
 // red-black sweep pairs per side of the V-cycle on the levels outside the fused coarse cycle
 constexpr int MG_SWEEPS = MG_NU;
 
+// Levels >= MG_GALERKIN_FROM carry the Galerkin operator R J P of the level above (27
+// coefficients per node) and are smoothed by damped Jacobi; the levels above keep the
+// rediscretised 7-point operator and red-black Gauss-Seidel.  The rediscretised coarse
+// operators of the Neumann-reduced system cost about half the convergence rate of the
+// preconditioned CG (0.30 against 0.17-0.20 per iteration in a CPU model of this cycle);
+// level 1 keeps the cheap 7-point form, which gives up little of that.  Measured on the
+// 257x257x513 sphere flow: 14.6 -> 9.5 CG iterations and 29.6 -> 21.6 ms of Newton solve
+// per time step.
+#ifndef MG_GAL_FROM
+#define MG_GAL_FROM 2
+#endif
+#ifndef MG_JAC_W
+#define MG_JAC_W 0.8f
+#endif
+constexpr int MG_GALERKIN_FROM = MG_GAL_FROM;
+constexpr float MG_JACOBI_W = MG_JAC_W;
+
 struct MgHierarchy {
     std::vector<MgLevel> L;
+    unsigned rap_stamp = ~0u;  // world geometry the Galerkin operators were built for
     int coarse_sweeps = 0;
     int fuse_l0 = -1;       // first level of the single-workgroup coarse cycle (-1: none)
     MgDev *dev = nullptr;   // device copy of the level descriptors
@@ -290,6 +314,199 @@ __device__ inline void mg_prolong_node(const Grid &gf, const Grid &gc, const uin
     if (m & 64) xf[u] = xu + s * (1.0f / (float)(1 << (ai + aj + ak)));
 }
 
+// ---- Galerkin levels ------------------------------------------------------------------------
+// a27[q nn + u], q = (di+1) + 3 (dj+1) + 9 (dk+1) (q = 13 the centre): the Laplacian part of
+// R J P at regular node u; the Boltzmann derivative stays on the diagonal, injected, in
+// idiag = 1 / (a27[13] - P).  Zero off the regular nodes, whose 26 neighbours all exist.
+__device__ inline long off27(const Grid &g, int q) {
+    int di = q % 3 - 1, dj = (q / 3) % 3 - 1, dk = q / 9 - 1;
+    return di + (long)g.ni * (dj + (long)g.nj * dk);
+}
+
+// (A x)(u) less the centre term
+__device__ inline float a27_offdiag(const Grid &g, const float *a27, const float *x, long u) {
+    float s = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++)
+        if (q != 13) s += a27[q * g.nn + u] * x[u + off27(g, q)];
+    return s;
+}
+
+// first damped-Jacobi sweep from x = 0: x_out = w b / diag at the regular nodes; the others
+// become 0 in both buffers (x_oth may hold face values of the last extension, or a node the
+// geometry has since made Dirichlet), so that prolongation reads 0 there
+template <typename TB>
+__device__ inline void jac_first_node(const Grid &g, const uint8_t *mask, const float *idiag, const TB *b,
+                                      float *x_out, float *x_oth, long u) {
+    if (mask[u] & 64) {
+        x_out[u] = MG_JACOBI_W * (float)b[u] * idiag[u];
+    } else {
+        x_out[u] = 0.0f;
+        x_oth[u] = 0.0f;
+    }
+}
+
+// damped-Jacobi sweep: x_out = x_in + w (b - A x_in) / diag at the regular nodes (the
+// others stay 0 in both buffers)
+template <typename TB>
+__device__ inline void jac_node(const Grid &g, const float *a27, const uint8_t *mask, const float *idiag,
+                                const float *x_in, const TB *b, float *x_out, long u) {
+    if (!(mask[u] & 64)) return;
+    float s = a27_offdiag(g, a27, x_in, u);
+    float du = idiag[u], xu = x_in[u];
+    x_out[u] = xu + MG_JACOBI_W * (((float)b[u] - s) * du - xu);
+}
+
+// r = b - A x at the regular nodes, 0 elsewhere
+template <typename TB>
+__device__ inline void res27_node(const Grid &g, const float *a27, const uint8_t *mask, const float *idiag,
+                                  const float *x, const TB *b, float *r, long u) {
+    float v = 0.0f;
+    if (mask[u] & 64) v = (float)b[u] - (a27_offdiag(g, a27, x, u) + x[u] / idiag[u]);
+    r[u] = v;
+}
+
+// trilinear interpolation of x_c at fine node (i, j, k): 1, 2, 4 or 8 coarse nodes (an odd
+// fine index has its upper coarse neighbour inside the coarse grid)
+__device__ inline float prolong_at(const Grid &gc, const float *xc, int i, int j, int k) {
+    int ai = i & 1, aj = j & 1, ak = k & 1;
+    long sj = gc.ni, sk = (long)gc.ni * gc.nj;
+    long b = (i >> 1) + sj * (j >> 1) + sk * (k >> 1);
+    float s = 0;
+    for (int c = 0; c <= ak; c++)
+        for (int bb = 0; bb <= aj; bb++)
+            for (int a = 0; a <= ai; a++) s += xc[b + a + sj * bb + sk * c];
+    return s * (1.0f / (float)(1 << (ai + aj + ak)));
+}
+
+// x_f += trilinear interpolation of x_c at every regular node (the Jacobi sweeps that follow
+// read both colours)
+__device__ inline void prolong_all_node(const Grid &gf, const Grid &gc, const uint8_t *mask_f, const float *xc,
+                                        float *xf, long u) {
+    int i, j, k;
+    node_ijk(gf, u, i, j, k);
+    uint8_t m = mask_f[u];
+    float xu = xf[u];
+    float c = prolong_at(gc, xc, i, j, k);
+    if (m & 64) xf[u] = xu + c;
+}
+
+// the regular coarse node whose value node K takes under the extension E (K itself, or the
+// end of its copy chain); -1 for a Dirichlet node or a chain that ends on one
+__device__ inline long ext_target(const Grid &g, const uint8_t *oid, long K, int i, int j, int k) {
+    int t = node_type(g, oid, K, i, j, k);
+    if (t == NT_REG) return K;
+    if (t == NT_DIR) return -1;
+    long c = chain_target(g, oid, K, i, j, k);
+    int ci, cj, ck;
+    node_ijk(g, c, ci, cj, ck);
+    return node_type(g, oid, c, ci, cj, ck) == NT_REG ? c : -1;
+}
+
+// The operator (Laplacian part) of the level above a Galerkin level: the rediscretised
+// 7-point one (a27 == nullptr: neighbour coefficients C, a Neumann neighbour folded into the
+// centre as k_mg_ops does) or its own Galerkin operator.
+struct FineOp {
+    Grid g;
+    const uint8_t *oid;
+    Coef6 C;
+    float dc;  // -2 (idx2 + idy2 + idz2)
+    const float *a27;
+};
+
+// row I of (1/8) P^T A_f P, P = trilinear interpolation after E on the coarse level, over the
+// regular nodes of both levels (the restriction of the cycle is (1/8) P^T: full weighting,
+// then the fold E^T)
+__device__ inline void rap_row(const FineOp &F, const Grid &gc, const uint8_t *oid_c, long I, float *out,
+                               long stride) {
+    int Ii, Ij, Ik;
+    node_ijk(gc, I, Ii, Ij, Ik);
+    double acc[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) acc[q] = 0.0;
+    if (node_type(gc, oid_c, I, Ii, Ij, Ik) == NT_REG) {
+        // a regular node with a non-regular coarse neighbour takes values through copy chains
+        // of nodes up to one step away, which reach fine nodes up to three steps from 2I
+        bool near = false;
+        for (int q = 0; q < 27 && !near; q++) {
+            int di = q % 3 - 1, dj = (q / 3) % 3 - 1, dk = q / 9 - 1;
+            long K = I + di + (long)gc.ni * (dj + (long)gc.nj * dk);
+            near = node_type(gc, oid_c, K, Ii + di, Ij + dj, Ik + dk) != NT_REG;
+        }
+        const int R = near ? 3 : 1;
+        const Grid &gf = F.g;
+        auto parent_target = [&](int pi, int pj, int pk) -> long {
+            long K = pi + (long)gc.ni * (pj + (long)gc.nj * pk);
+            return near ? ext_target(gc, oid_c, K, pi, pj, pk) : K;
+        };
+        for (int fk = max(0, 2 * Ik - R); fk <= min(gf.nk - 1, 2 * Ik + R); fk++)
+            for (int fj = max(0, 2 * Ij - R); fj <= min(gf.nj - 1, 2 * Ij + R); fj++)
+                for (int fi = max(0, 2 * Ii - R); fi <= min(gf.ni - 1, 2 * Ii + R); fi++) {
+                    long f = fi + (long)gf.ni * (fj + (long)gf.nj * fk);
+                    if (node_type(gf, F.oid, f, fi, fj, fk) != NT_REG) continue;
+                    // weight of I in the interpolation at f
+                    int ai = fi & 1, aj = fj & 1, ak = fk & 1;
+                    double w = 1.0 / (double)(1 << (ai + aj + ak));
+                    double pf = 0;
+                    for (int c = 0; c <= ak; c++)
+                        for (int b = 0; b <= aj; b++)
+                            for (int a = 0; a <= ai; a++)
+                                if (parent_target((fi >> 1) + a, (fj >> 1) + b, (fk >> 1) + c) == I) pf += w;
+                    if (pf == 0) continue;
+                    // the fine row at f
+                    const int nq = F.a27 ? 27 : 7;
+                    for (int e = 0; e < nq; e++) {
+                        int gi, gj, gk;
+                        double av;
+                        if (F.a27) {
+                            gi = fi + e % 3 - 1;
+                            gj = fj + (e / 3) % 3 - 1;
+                            gk = fk + e / 9 - 1;
+                            av = F.a27[e * gf.nn + f];
+                        } else {
+                            // e = 0: centre; 1..6: the six neighbours (-z, -y, -x, +x, +y, +z)
+                            const int di[7] = {0, 0, 0, -1, 1, 0, 0}, dj[7] = {0, 0, -1, 0, 0, 1, 0},
+                                      dk[7] = {0, -1, 0, 0, 0, 0, 1};
+                            gi = fi + di[e];
+                            gj = fj + dj[e];
+                            gk = fk + dk[e];
+                            long gg = gi + (long)gf.ni * (gj + (long)gf.nj * gk);
+                            if (e == 0) {
+                                double dgv = F.dc;
+                                for (int d = 0; d < 6; d++) {
+                                    long v = f + nb_offset(gf, d);
+                                    const int ddi[6] = {0, 0, -1, 1, 0, 0}, ddj[6] = {0, -1, 0, 0, 1, 0},
+                                              ddk[6] = {-1, 0, 0, 0, 0, 1};
+                                    if (node_type(gf, F.oid, v, fi + ddi[d], fj + ddj[d], fk + ddk[d]) == NT_NEU)
+                                        dgv += F.C.c[d];
+                                }
+                                av = dgv;
+                            } else {
+                                av = node_type(gf, F.oid, gg, gi, gj, gk) == NT_REG ? F.C.c[e - 1] : 0.0;
+                            }
+                        }
+                        if (av == 0) continue;
+                        // interpolation weights of g onto coarse nodes J near I
+                        int bi = gi & 1, bj = gj & 1, bk = gk & 1;
+                        double wg = 1.0 / (double)(1 << (bi + bj + bk));
+                        for (int c = 0; c <= bk; c++)
+                            for (int b = 0; b <= bj; b++)
+                                for (int a = 0; a <= bi; a++) {
+                                    int pi = (gi >> 1) + a, pj = (gj >> 1) + b, pk = (gk >> 1) + c;
+                                    long J = parent_target(pi, pj, pk);
+                                    if (J < 0) continue;
+                                    int Ji, Jj, Jk;
+                                    node_ijk(gc, J, Ji, Jj, Jk);
+                                    int dI = Ji - Ii, dJ = Jj - Ij, dK = Jk - Ik;
+                                    if (dI < -1 || dI > 1 || dJ < -1 || dJ > 1 || dK < -1 || dK > 1) continue;
+                                    acc[(dI + 1) + 3 * (dJ + 1) + 9 * (dK + 1)] += pf * av * wg;
+                                }
+                    }
+                }
+    }
+    for (int q = 0; q < 27; q++) out[q * stride] = (float)(acc[q] / 8.0);
+}
+
 // ---- per-level kernels (one node per thread, XCD-remapped workgroups) ---------------------
 template <typename TB>
 __global__ void __launch_bounds__(256)
@@ -383,6 +600,53 @@ __global__ void __launch_bounds__(256)
     if (t < smooth_count(gf)) mg_prolong_node(gf, gc, mask_f, xc, xf, t);
 }
 
+// Galerkin levels
+__global__ void __launch_bounds__(256) k_mg_rap(FineOp F, Grid gc, const uint8_t *oid_c, float *a27) {
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < gc.nn) rap_row(F, gc, oid_c, u, a27 + u, gc.nn);
+}
+
+template <typename TB>
+__global__ void __launch_bounds__(256) k_mg_jac_first(Grid g, const uint8_t *mask, const float *idiag, const TB *b,
+                                                                                 float *x_out, float *x_oth, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < g.nn) jac_first_node(g, mask, idiag, b, x_out, x_oth, u);
+}
+
+template <typename TB>
+__global__ void __launch_bounds__(256)
+    k_mg_jac(Grid g, const float *a27, const uint8_t *mask, const float *idiag, const float *x_in, const TB *b,
+             float *x_out, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < g.nn) jac_node(g, a27, mask, idiag, x_in, b, x_out, u);
+}
+
+template <typename TB>
+__global__ void __launch_bounds__(256)
+    k_mg_res27(Grid g, const float *a27, const uint8_t *mask, const float *idiag, const float *x, const TB *b,
+               float *r, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < g.nn) res27_node(g, a27, mask, idiag, x, b, r, u);
+}
+
+__global__ void __launch_bounds__(256)
+    k_mg_prolong_all(Grid gf, Grid gc, const uint8_t *mask_f, const float *xc, float *xf, const DevScalars *sc) {
+    if (sc->done) return;
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u < gf.nn) prolong_all_node(gf, gc, mask_f, xc, xf, u);
+}
+
+// reciprocal diagonal of a Galerkin level: 1 / (a27 centre - P) (the mask from k_mg_ops)
+__global__ void __launch_bounds__(256)
+    k_mg_gal_diag(Grid g, const float *a27, const uint8_t *mask, const float *P, float *idiag) {
+    long u = xcd_block() * blockDim.x + threadIdx.x;
+    if (u >= g.nn) return;
+    idiag[u] = (mask[u] & 64) ? (float)(1.0 / ((double)a27[13 * g.nn + u] - (double)P[u])) : 1.0f;
+}
+
 // ---- the coarse levels in one workgroup -------------------------------------------------
 // Levels l0..nl-1 (each at most MG_FUSE_NODES nodes) run the down leg, the coarsest solve and
 // the up leg of the V-cycle inside one launch of 1024 threads, a workgroup barrier between
@@ -394,6 +658,8 @@ struct MgDev {
     const uint8_t *oid, *mask;
     const float *idiag;
     float *x, *r, *b;
+    const float *a27;  // Galerkin levels (else nullptr)
+    float *x2;
 };
 
 #ifndef MG_FUSE_MAX
@@ -406,12 +672,32 @@ __global__ void __launch_bounds__(MG_FUSE_THREADS)
     k_mg_coarse(const MgDev *L, int l0, int nl, int sweeps, const DevScalars *sc) {
     if (sc->done) return;
     const long T = MG_FUSE_THREADS;
+    // n damped-Jacobi sweeps of a Galerkin level: from x = 0 (into x2, then alternating) or
+    // from x; an even n ends in x
+    auto jacobi = [&](const MgDev &f, int n, bool from_zero) {
+        for (int s = 0; s < n; s++) {
+            if (s == 0 && from_zero) {
+                for (long u = threadIdx.x; u < f.g.nn; u += T) jac_first_node(f.g, f.mask, f.idiag, f.b, f.x2, f.x, u);
+            } else {
+                const bool in2 = (s & 1) != 0;  // odd sweeps read x2 (both starts)
+                const float *xi = in2 ? f.x2 : f.x;
+                float *xo = in2 ? f.x : f.x2;
+                for (long u = threadIdx.x; u < f.g.nn; u += T) jac_node(f.g, f.a27, f.mask, f.idiag, xi, f.b, xo, u);
+            }
+            __syncthreads();
+        }
+    };
     for (int l = l0; l < nl - 1; l++) {
         const MgDev &f = L[l], &c = L[l + 1];
-        for (long u = threadIdx.x; u < f.g.nn; u += T) mg_pre_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, u);
-        __syncthreads();
-        for (long u = threadIdx.x; u < f.g.nn; u += T)
-            mg_residual_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, f.r, u);
+        if (f.a27) {
+            jacobi(f, 2, true);
+            for (long u = threadIdx.x; u < f.g.nn; u += T) res27_node(f.g, f.a27, f.mask, f.idiag, f.x, f.b, f.r, u);
+        } else {
+            for (long u = threadIdx.x; u < f.g.nn; u += T) mg_pre_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, u);
+            __syncthreads();
+            for (long u = threadIdx.x; u < f.g.nn; u += T)
+                mg_residual_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, f.r, u);
+        }
         __syncthreads();
         for (long u = threadIdx.x; u < c.g.nn; u += T) mg_restrict_node(f.g, c.g, f.r, c.r, u);
         __syncthreads();
@@ -420,35 +706,46 @@ __global__ void __launch_bounds__(MG_FUSE_THREADS)
     }
     {
         const MgDev &c = L[nl - 1];
-        auto half_sweep = [&](int color) {
-            for (long t = threadIdx.x; t < smooth_count(c.g); t += T)
-                mg_smooth_node(c.g, c.C, c.mask, c.idiag, c.x, c.b, color, t);
+        if (c.a27) {
+            jacobi(c, 2 * sweeps, true);
+        } else {
+            auto half_sweep = [&](int color) {
+                for (long t = threadIdx.x; t < smooth_count(c.g); t += T)
+                    mg_smooth_node(c.g, c.C, c.mask, c.idiag, c.x, c.b, color, t);
+                __syncthreads();
+            };
+            for (long u = threadIdx.x; u < c.g.nn; u += T) mg_pre_node(c.g, c.C, c.mask, c.idiag, c.x, c.b, u);
             __syncthreads();
-        };
-        for (long u = threadIdx.x; u < c.g.nn; u += T) mg_pre_node(c.g, c.C, c.mask, c.idiag, c.x, c.b, u);
-        __syncthreads();
-        // (red, black) pairs, then as many (black, red) pairs: a symmetric coarsest solve
-        for (int it = 1; it < sweeps; it++) {
-            half_sweep(0);
-            half_sweep(1);
-        }
-        for (int it = 0; it < sweeps; it++) {
-            half_sweep(1);
-            half_sweep(0);
+            // (red, black) pairs, then as many (black, red) pairs: a symmetric coarsest solve
+            for (int it = 1; it < sweeps; it++) {
+                half_sweep(0);
+                half_sweep(1);
+            }
+            for (int it = 0; it < sweeps; it++) {
+                half_sweep(1);
+                half_sweep(0);
+            }
         }
     }
     for (int l = nl - 2; l >= l0; l--) {
         const MgDev &f = L[l], &c = L[l + 1];
         for (long u = threadIdx.x; u < c.g.nn; u += T) mg_extend_node(c.g, c.oid, c.x, u);
         __syncthreads();
-        for (long t = threadIdx.x; t < smooth_count(f.g); t += T) mg_prolong_node(f.g, c.g, f.mask, c.x, f.x, t);
-        __syncthreads();
-        for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
-            mg_smooth_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, 1, t);
-        __syncthreads();
-        for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
-            mg_smooth_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, 0, t);
-        __syncthreads();
+        if (f.a27) {
+            for (long u = threadIdx.x; u < f.g.nn; u += T) prolong_all_node(f.g, c.g, f.mask, c.x, f.x, u);
+            __syncthreads();
+            jacobi(f, 2, false);
+        } else {
+            for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
+                mg_prolong_node(f.g, c.g, f.mask, c.x, f.x, t);
+            __syncthreads();
+            for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
+                mg_smooth_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, 1, t);
+            __syncthreads();
+            for (long t = threadIdx.x; t < smooth_count(f.g); t += T)
+                mg_smooth_node(f.g, f.C, f.mask, f.idiag, f.x, f.b, 0, t);
+            __syncthreads();
+        }
     }
 }
 
@@ -477,6 +774,8 @@ void mg_destroy(MgHierarchy *H) {
         hipFree(l.idiag);
         hipFree(l.x);
         hipFree(l.r);
+        hipFree(l.a27);
+        hipFree(l.x2);
         if (!l.owned) continue;
         hipFree(l.oid);
         hipFree(l.mask);
@@ -508,6 +807,7 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
         c.g.nn = (long)c.g.ni * c.g.nj * c.g.nk;
         for (int a = 0; a < 3; a++) c.g.dh[a] = 2 * f.dh[a];
         c.owned = true;
+        c.gal = (int)H->L.size() >= MG_GALERKIN_FROM;
         H->L.push_back(c);
     }
     hipError_t e = hipSuccess;
@@ -517,6 +817,13 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
         if (e == hipSuccess) e = hipMalloc(&c.idiag, n * sizeof(float));
         if (e == hipSuccess) e = hipMalloc(&c.x, n * sizeof(float));
         if (e == hipSuccess) e = hipMalloc(&c.r, n * sizeof(float));
+        if (c.gal) {
+            // x2 is read (times a zero coefficient) at nodes no sweep writes: it starts at 0
+            if (e == hipSuccess) e = hipMalloc(&c.a27, 27 * n * sizeof(float));
+            if (e == hipSuccess) e = hipMalloc(&c.x2, n * sizeof(float));
+            if (e == hipSuccess) e = hipMemset(c.x2, 0, n * sizeof(float));
+            if (e == hipSuccess) e = hipMemset(c.x, 0, n * sizeof(float));
+        }
         if (!c.owned) continue;
         if (e == hipSuccess) e = hipMalloc(&c.oid, n);
         if (e == hipSuccess) e = hipMalloc(&c.mask, n);
@@ -555,6 +862,8 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
             d.x = v.x;
             d.r = v.r;
             d.b = v.b;
+            d.a27 = v.a27;
+            d.x2 = v.x2;
         }
         e = hipMalloc(&H->dev, sizeof(MgDev) * nl);
         if (e == hipSuccess) e = hipMemcpy(H->dev, hd.data(), sizeof(MgDev) * nl, hipMemcpyHostToDevice);
@@ -568,8 +877,11 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
 }
 
 // Level operators for the current Newton step: level 0's mask, jr and P come fresh from
-// k_nrs_F.
-int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, hipStream_t st) {
+// k_nrs_F.  The Laplacian parts of the Galerkin operators depend on the geometry only and
+// are rebuilt when its stamp changes; their diagonals take each step's injected P.
+int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, unsigned geom_stamp,
+             hipStream_t st) {
+    const bool rap = H->rap_stamp != geom_stamp;
     for (size_t l = 0; l < H->L.size(); l++) {
         MgLevel &c = H->L[l];
         c.Q = level_params(Q0, c.g);
@@ -583,8 +895,15 @@ int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const doubl
         else
             k_mg_inject<float><<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, f.boltz, c.oid, c.boltz);
         k_mg_ops<<<nbk8(c.g.nn), 256, 0, st>>>(c.Q, c.oid, c.boltz, c.mask, c.idiag);
+        if (!c.gal) continue;
+        if (rap) {
+            FineOp F{f.g, f.oid, coef6(f.Q), (float)(-2.0 * (f.Q.idx2 + f.Q.idy2 + f.Q.idz2)), f.gal ? f.a27 : nullptr};
+            k_mg_rap<<<nbk8(c.g.nn), 256, 0, st>>>(F, c.g, c.oid, c.a27);
+        }
+        k_mg_gal_diag<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.a27, c.mask, c.boltz, c.idiag);
     }
     PIC_LAUNCH_CHECK();
+    H->rap_stamp = geom_stamp;
     return PIC_OK;
 }
 
@@ -593,6 +912,14 @@ template <typename TB>
 static void down(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScalars *sc) {
     MgLevel &f = H->L[l], &c = H->L[l + 1];
     Coef6 C = coef6(f.Q);
+    if (f.gal) {  // two damped-Jacobi sweeps from 0 (into x2, then x)
+        k_mg_jac_first<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.mask, f.idiag, b, f.x2, f.x, sc);
+        k_mg_jac<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.a27, f.mask, f.idiag, f.x2, b, f.x, sc);
+        k_mg_res27<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.a27, f.mask, f.idiag, f.x, b, f.r, sc);
+        k_mg_restrict<<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.r, c.r, sc);
+        k_mg_fold<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.mask, c.r, c.b, sc);
+        return;
+    }
     k_mg_pre<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, sc);
     for (int it = 1; it < MG_SWEEPS; it++) {  // further (red, black) pairs
         k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
@@ -619,6 +946,12 @@ static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScal
     MgLevel &f = H->L[l], &c = H->L[l + 1];
     Coef6 C = coef6(f.Q);
     k_mg_extend<<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.oid, c.x, sc);
+    if (f.gal) {  // x += P x_c at every regular node, then two damped-Jacobi sweeps
+        k_mg_prolong_all<<<nbk8(f.g.nn), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
+        k_mg_jac<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.a27, f.mask, f.idiag, f.x, b, f.x2, sc);
+        k_mg_jac<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, f.a27, f.mask, f.idiag, f.x2, b, f.x, sc);
+        return;
+    }
     k_mg_prolong<<<smooth_blocks(f.g), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
     k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
     for (int it = 1; it < MG_SWEEPS; it++) {  // the pre-smoothing pairs in reverse: (black, red)
@@ -660,6 +993,14 @@ int mg_vcycle(MgHierarchy *H, const double *b0, const float *b0f, float **x0, do
     {
         MgLevel &c = H->L[nl - 1];
         Coef6 C = coef6(c.Q);
+        if (c.gal) {  // 2 coarse_sweeps damped-Jacobi sweeps from 0, ending in x
+            k_mg_jac_first<float><<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.mask, c.idiag, c.b, c.x2, c.x, sc);
+            for (int it = 1; it < 2 * H->coarse_sweeps; it++) {
+                const bool in2 = (it & 1) != 0;
+                k_mg_jac<float><<<nbk8(c.g.nn), 256, 0, st>>>(c.g, c.a27, c.mask, c.idiag, in2 ? c.x2 : c.x, c.b,
+                                                               in2 ? c.x : c.x2, sc);
+            }
+        } else {
         k_mg_pre<float><<<nbk8(c.g.nn), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, sc);
         // symmetric sweeps: (red, black) pairs, then as many (black, red) pairs
         for (int it = 1; it < H->coarse_sweeps; it++) {
@@ -669,6 +1010,7 @@ int mg_vcycle(MgHierarchy *H, const double *b0, const float *b0f, float **x0, do
         for (int it = 0; it < H->coarse_sweeps; it++) {
             k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, 1, sc);
             k_mg_smooth<float><<<smooth_blocks(c.g), 256, 0, st>>>(c.g, C, c.mask, c.idiag, c.x, c.b, 0, sc);
+        }
         }
     }
     for (int l = nl - 2; l >= 1; l--) up<float>(H, l, H->L[l].b, st, sc);

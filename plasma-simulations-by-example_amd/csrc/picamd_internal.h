@@ -193,6 +193,7 @@ struct pic_world {
     int partials_cap = 0;
     double *pe_part = nullptr;        // per-block |E|^2 V partials of the last computeEF
     bool ef_valid = false;            // ef32/ef64 hold computeEF of the current phi
+    unsigned geom_stamp = 0;          // bumped whenever object_id changes (sphere, inlet)
     bool pe_valid = false;
     // timers
     bool timing = false;

@@ -397,6 +397,7 @@ int pic_world_add_sphere(pic_world *w, const double c[3], double r, double phi_s
                                                                c[2], r * r, phi_s);
     PIC_LAUNCH_CHECK();
     w->ef_valid = false;
+    w->geom_stamp++;
     return PIC_OK;
 }
 
@@ -406,6 +407,7 @@ int pic_world_add_inlet(pic_world *w) {
     k_add_inlet<<<nblocks(n, 256), 256, 0, w->stream>>>(w->g, w->object_id, w->phi);
     PIC_LAUNCH_CHECK();
     w->ef_valid = false;
+    w->geom_stamp++;
     return PIC_OK;
 }
 
