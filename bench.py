@@ -56,9 +56,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--T-ion", type=float, default=1000.0)
-    ap.add_argument("--horizon", type=int, default=100,
+    ap.add_argument("--horizon", type=int, default=None,
                     help="steps in all (untimed beyond warmup+steps) before the mixed-vs-FP64 "
-                         "field and history comparison")
+                         "field and history comparison (default 100; 0 with --no-fp64)")
     ap.add_argument("--push-only", action="store_true",
                     help="profiling mode: the same particle workload, pushes only (no solve), "
                          "then one Field_::updateAverage pass (a plain streaming kernel of known "
@@ -337,7 +337,7 @@ def main():
     if args.push_only:
         push_only(P, args, grid)
         return
-    horizon = 0 if args.no_fp64 else args.horizon
+    horizon = args.horizon if args.horizon is not None else (0 if args.no_fp64 else 100)
     mixed = run_gpu(P, grid, args.particles, P.MIXED, args.steps, args.warmup, args.T_ion,
                     want_fields=True, horizon=horizon)
     log(f"[bench] mixed: {mixed['particle_steps_per_s']:.4e} particle-steps/s, "

@@ -73,7 +73,7 @@ constexpr int MG_SWEEPS = MG_NU;
 #define MG_GAL_FROM 2
 #endif
 #ifndef MG_JAC_W
-#define MG_JAC_W 0.8f
+#define MG_JAC_W 0.9f
 #endif
 constexpr int MG_GALERKIN_FROM = MG_GAL_FROM;
 constexpr float MG_JACOBI_W = MG_JAC_W;
@@ -650,8 +650,8 @@ __global__ void __launch_bounds__(256)
 // ---- the coarse levels in one workgroup -------------------------------------------------
 // Levels l0..nl-1 (each at most MG_FUSE_NODES nodes) run the down leg, the coarsest solve and
 // the up leg of the V-cycle inside one launch of 1024 threads, a workgroup barrier between
-// steps; each step is the same per-node code as the per-level kernels.  Replaces some 70
-// launches of a few microseconds each per V-cycle.
+// steps; each step is the same per-node code as the per-level kernels.  Replaces some 40
+// launches of a few microseconds each per V-cycle (the coarsest solve's sweeps among them).
 struct MgDev {
     Grid g;
     Coef6 C;
@@ -662,10 +662,13 @@ struct MgDev {
     float *x2;
 };
 
+// Levels of at most MG_FUSE_MAX nodes (9x9x17 and below at 257x257x513).  With the 27-point
+// Galerkin sweeps, 17x17x33 (9537 nodes) runs faster as per-level kernels of 40 workgroups
+// than inside the single workgroup (about 1 ms per time step).
 #ifndef MG_FUSE_MAX
-#define MG_FUSE_MAX 12000
+#define MG_FUSE_MAX 2000
 #endif
-constexpr long MG_FUSE_NODES = MG_FUSE_MAX;  // about 12 nodes per thread and step
+constexpr long MG_FUSE_NODES = MG_FUSE_MAX;  // about 2 nodes per thread and step
 constexpr int MG_FUSE_THREADS = 1024;
 
 __global__ void __launch_bounds__(MG_FUSE_THREADS)

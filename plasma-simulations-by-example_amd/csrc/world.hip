@@ -390,6 +390,9 @@ int pic_world_set_extents(pic_world *w, const double x0[3], const double xm[3]) 
 int pic_world_add_sphere(pic_world *w, const double c[3], double r, double phi_s) {
     PIC_CHECK(w && c, PIC_ERR_ARG, "null argument");
     PIC_CHECK(w->extents_set, PIC_ERR_STATE, "setExtents must precede addSphere");
+    // the same sphere again (a new potential only) leaves object_id as it was
+    const bool same = w->sphere.on && w->sphere.c[0] == c[0] && w->sphere.c[1] == c[1] &&
+                      w->sphere.c[2] == c[2] && w->sphere.r2 == r * r;
     w->sphere.on = 1;
     for (int q = 0; q < 3; q++) w->sphere.c[q] = c[q];
     w->sphere.r2 = r * r;
@@ -397,7 +400,7 @@ int pic_world_add_sphere(pic_world *w, const double c[3], double r, double phi_s
                                                                c[2], r * r, phi_s);
     PIC_LAUNCH_CHECK();
     w->ef_valid = false;
-    w->geom_stamp++;
+    if (!same) w->geom_stamp++;
     return PIC_OK;
 }
 
