@@ -439,7 +439,7 @@ __device__ inline void deposit_global(const Grid &g, double *den, const PT &p) {
     deposit_global64(g, den, i, j, k, di, dj, dk, w);
 }
 
-template <typename T>
+template <typename T, int WN = pic::WN>
 __device__ inline void deposit_lds(double *sW, int wi, int wj, int wk, T ox, T oy, T oz, T w) {
     int b = wi + WN * (wj + WN * wk);
     T ax = T(1) - ox, ay = T(1) - oy, az = T(1) - oz;
@@ -637,13 +637,20 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_starts(Grid g, PlanTmp T,
 }
 
 // push workgroups per tile: one per PUSH_CHUNK of its input particles, at least one
-__global__ void __launch_bounds__(PLAN_THREADS) k_plan_chunks(Grid g, PlanTmp T) {
+// and which instantiation pushes each tile: the wide-rim one when at least 32 of its
+// particles, and 1/32 of them, deposited outside the window in the last push (force: 1 all
+// tiles, -1 none); the far counts restart from 0
+__global__ void __launch_bounds__(PLAN_THREADS)
+    k_plan_chunks(Grid g, PlanTmp T, uint32_t *far, uint32_t *wide, int force) {
     __shared__ unsigned long long sh[PLAN_THREADS / 64];
     int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
     uint32_t c = 0;
     if (t < g.ntiles) {
         c = max(1u, (T.need[t] + PUSH_CHUNK - 1) / PUSH_CHUNK);
         T.nchunk[t] = c;
+        uint32_t f = far[t];
+        far[t] = 0;
+        wide[t] = force > 0 ? 1u : force < 0 ? 0u : (f >= 32u && (unsigned long long)f * 32u >= T.need[t]) ? 1u : 0u;
     }
     unsigned long long a = block_sum_u64(c, sh);
     if (threadIdx.x == 0) T.blk2[blockIdx.x] = a;
@@ -682,6 +689,8 @@ struct PushOut {
     double *den_extra;
     double *diag_part;  // [NDIAG_SLOTS][gridDim.x] per-block partials, reduced on demand
     uint32_t *err;
+    uint32_t *far;          // per tile: particles deposited outside the window this push
+    const uint32_t *wide;   // per tile: pushed by the wide-rim instantiation
 };
 
 // slot of an arrival in the inflow segment of tile dtile, -1 when the segment is full
@@ -706,19 +715,25 @@ __device__ inline long staging_slot(bool need, const PartView &out, uint32_t *er
     return -1L;
 }
 
-template <int PREC>
+// RIM: the window's rim in cells, HALO or HALO_WIDE.  Both instantiations are launched over
+// the same workgroups; each pushes the tiles the plan gave it (the staging list goes to the
+// HALO one) and returns at once elsewhere, leaving that workgroup's partials to the other.
+template <int PREC, int RIM>
 __global__ void __launch_bounds__(PUSH_THREADS)
     k_push(PushParams P, PartView in, PartView out, const float4 *__restrict__ ef32,
            const double4 *__restrict__ ef64, PushOut O) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    constexpr bool WIDE = RIM != HALO;
+    constexpr int WNR = TILE + 1 + 2 * RIM;  // window nodes per edge
+    constexpr int WNODESR = WNR * WNR * WNR;
     const Grid &g = P.g;
-    __shared__ uint32_t sOwn;
+    __shared__ uint32_t sOwn, sFar;
     __shared__ double sRed[NDIAG_SLOTS][PUSH_THREADS / 64];
     // LDS: E tile (3 float or 3 double planes), the FP64 deposit window, and the output bin
     // segments of the 27 tiles around this one (start, inflow offset, capacity)
     __shared__ __attribute__((aligned(16))) unsigned char
         sRaw[(PREC == PIC_MIXED ? TNODES * 12 : TNODES * 24)];
-    __shared__ double sW[WNODES];
+    __shared__ double sW[WNODESR];
     __shared__ uint32_t sNb[3][27];
 
     // per-thread diagnostic sums in the particle arithmetic (a thread sees a few dozen
@@ -759,6 +774,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         const uint64_t tc = O.chunk_map[wg];
         const int t = (int)(uint32_t)tc;
         const uint32_t chunk = (uint32_t)(tc >> 32);
+        if ((O.wide[t] != 0u) != WIDE) return;  // the other instantiation's tile
         BinSeg bs = bin_of(in, t);
         const uint32_t cnt_tile = bs.c0 + bs.c1;
         const bool split = cnt_tile > (uint32_t)PUSH_CHUNK;  // several workgroups share tile t
@@ -791,7 +807,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                 pl[2 * TNODES + n] = e.z;
             }
         }
-        for (int n = tid; n < WNODES; n += PUSH_THREADS) sW[n] = 0.0;
+        for (int n = tid; n < WNODESR; n += PUSH_THREADS) sW[n] = 0.0;
         if (tid < 27) {
             int nx = tx + tid % 3 - 1, ny = ty + (tid / 3) % 3 - 1, nz = tz + tid / 9 - 1;
             bool in = nx >= 0 && nx < g.ntx && ny >= 0 && ny < g.nty && nz >= 0 && nz < g.ntz;
@@ -800,7 +816,10 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             sNb[1][tid] = in ? out.bin_off[nt] : 0u;
             sNb[2][tid] = in ? out.bin_cap[nt] : 0u;
         }
-        if (tid == 0) sOwn = 0;
+        if (tid == 0) {
+            sOwn = 0;
+            sFar = 0;
+        }
         __syncthreads();
         const long own0 = out.bin_start[t];  // own segment of this tile in the output buffer
         const uint32_t own_cap = out.bin_off[t];
@@ -840,9 +859,9 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                 }
                 if (alive) {
                     tally(p);
-                    int a = ci - c0x + HALO, b = cj - c0y + HALO, c = ck - c0z + HALO;
-                    if (a >= 0 && a < TILE + 2 * HALO && b >= 0 && b < TILE + 2 * HALO && c >= 0 &&
-                        c < TILE + 2 * HALO) {
+                    int a = ci - c0x + RIM, b = cj - c0y + RIM, c = ck - c0z + RIM;
+                    if (a >= 0 && a < TILE + 2 * RIM && b >= 0 && b < TILE + 2 * RIM && c >= 0 &&
+                        c < TILE + 2 * RIM) {
                         wi = a;
                         wj = b;
                         wk = c;
@@ -870,12 +889,14 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             if (key >= 0 && kg.before == 0) lbase = atomicAdd(&out.bin_in[key], kg.count);
             if (wi >= 0) {
                 if constexpr (PREC == PIC_MIXED)
-                    deposit_lds<float>(sW, wi, wj, wk, p.ox, p.oy, p.oz, p.w);
+                    deposit_lds<float, WNR>(sW, wi, wj, wk, p.ox, p.oy, p.oz, p.w);
                 else
-                    deposit_lds<double>(sW, wi, wj, wk, di, dj, dk, p.w);
+                    deposit_lds<double, WNR>(sW, wi, wj, wk, di, dj, dk, p.w);
             } else if (alive) {
                 deposit_global(g, O.den_extra, p);
             }
+            const unsigned long long fm = __ballot(alive && wi < 0);  // far deposits, counted
+            if (fm && (int)__lane_id() == __ffsll((long long)fm) - 1) atomicAdd(&sFar, (uint32_t)__popcll(fm));
             long idx = -1;
             if (m) {
                 uint32_t slot = (uint32_t)__builtin_amdgcn_readlane((int)sbase, sleader) +
@@ -898,31 +919,43 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             if (idx >= 0) store_p(out, idx, p);
         }
         __syncthreads();
+        if (tid == 0 && sFar) atomicAdd(&O.far[t], sFar);
+        // window node n outside the tile's window slot (further chunks of a split tile: all of
+        // them; wide rim: the nodes beyond HALO cells) goes to the far-deposit array
+        auto add_far = [&](int n, double v) {
+            if (v == 0.0) return;
+            int gi = c0x - RIM + n % WNR, gj = c0y - RIM + (n / WNR) % WNR, gk = c0z - RIM + n / (WNR * WNR);
+            if (gi >= 0 && gi < g.ni && gj >= 0 && gj < g.nj && gk >= 0 && gk < g.nk)
+                atomicAdd(&O.den_extra[gi + sj * gj + sk * gk], v);
+        };
         if (chunk == 0) {
             // flush the deposit window into the tile's window slot
-            if (PREC == PIC_MIXED) {
-                float *dst = O.win32 + (long)t * WNODES;
-                for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = (float)sW[n];
-            } else {
-                double *dst = O.win64 + (long)t * WNODES;
-                for (int n = tid; n < WNODES; n += PUSH_THREADS) dst[n] = sW[n];
+            for (int n = tid; n < WNODESR; n += PUSH_THREADS) {
+                int m = n;  // index in the slot's HALO-rim window
+                if constexpr (WIDE) {
+                    constexpr int D = RIM - HALO;
+                    int a = n % WNR - D, b = (n / WNR) % WNR - D, c = n / (WNR * WNR) - D;
+                    if (a < 0 || a >= WN || b < 0 || b >= WN || c < 0 || c >= WN) {
+                        add_far(n, sW[n]);
+                        continue;
+                    }
+                    m = a + WN * (b + WN * c);
+                }
+                if constexpr (PREC == PIC_MIXED)
+                    O.win32[(long)t * WNODES + m] = (float)sW[n];
+                else
+                    O.win64[(long)t * WNODES + m] = sW[n];
             }
             if (tid == 0) {
                 O.win_used[t] = 1;
                 if (!split) out.bin_own[t] = sOwn;
             }
         } else {
-            // further chunks of a split tile add their window to the far-deposit array
-            for (int n = tid; n < WNODES; n += PUSH_THREADS) {
-                double v = sW[n];
-                if (v == 0.0) continue;
-                int gi = c0x - HALO + n % WN, gj = c0y - HALO + (n / WN) % WN, gk = c0z - HALO + n / (WN * WN);
-                if (gi >= 0 && gi < g.ni && gj >= 0 && gj < g.nj && gk >= 0 && gk < g.nk)
-                    atomicAdd(&O.den_extra[gi + sj * gj + sk * gk], v);
-            }
+            for (int n = tid; n < WNODESR; n += PUSH_THREADS) add_far(n, sW[n]);
         }
     } else {
         // ------------------------------------------------ unbinned staging list
+        if constexpr (WIDE) return;  // pushed by the HALO instantiation
         uint32_t ns = min(*in.stage_count, (uint32_t)in.cap_stage);
         uint32_t nthreads = PUSH_STAGE_WGS * PUSH_THREADS;
         uint32_t first = blockIdx.x * PUSH_THREADS + tid;
@@ -1559,7 +1592,10 @@ static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist,
     k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk, nblk, nullptr);
     k_plan_starts<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, vo);
     if (mode == 0) {  // push workgroups per tile
-        k_plan_chunks<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T);
+        // PIC_PUSH_WIDE=all / off forces the wide-rim push for every tile / for none (tests)
+        const char *fw = getenv("PIC_PUSH_WIDE");
+        const int force = !fw ? 0 : !strcmp(fw, "all") ? 1 : !strcmp(fw, "off") ? -1 : 0;
+        k_plan_chunks<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->far_tile, s->far_tile + g.ntiles, force);
         k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk2, nblk, T.total);
         k_plan_map<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->chunk_map, s->chunk_cap);
     }
@@ -1718,6 +1754,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMalloc(&s->den_ave, sizeof(double) * g.nn));
     TRY(hipMalloc(&s->win, wbytes));
     TRY(hipMalloc(&s->win_used, g.ntiles));
+    TRY(hipMalloc(&s->far_tile, sizeof(uint32_t) * 2 * (size_t)g.ntiles));
     TRY(hipMalloc(&s->plan_mem, plan_bytes(g)));
     TRY(hipMalloc(&s->diag_dev, sizeof(double) * 8));
     if ((rc = ensure_push_scratch(s))) goto fail;
@@ -1725,6 +1762,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMemsetAsync(s->np_dev, 0, sizeof(long), w->stream));
     TRY(hipMemsetAsync(s->den, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->den_extra, 0, sizeof(double) * g.nn, w->stream));
+    TRY(hipMemsetAsync(s->far_tile, 0, sizeof(uint32_t) * 2 * (size_t)g.ntiles, w->stream));
     TRY(hipMemsetAsync(s->den_ave, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->win_used, 0, g.ntiles, w->stream));
     TRY(hipMemsetAsync(s->plan_mem, 0, plan_bytes(g), w->stream));  // staged counts start at 0
@@ -1753,6 +1791,7 @@ int pic_species_destroy(pic_species *s) {
     hipFree(s->den_ave);
     hipFree(s->win);
     hipFree(s->win_used);
+    hipFree(s->far_tile);
     hipFree(s->plan_mem);
     hipFree(s->diag_dev);
     hipFree(s->diag_part);
@@ -1983,15 +2022,22 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     O.den_extra = s->den_extra;
     O.diag_part = s->diag_part;
     O.err = err_flag(s);
+    O.far = s->far_tile;
+    O.wide = s->far_tile + g.ntiles;
     // workgroups for the tiles (the plan's map decides which run) + the staging workgroups
     unsigned grid = (unsigned)(push_bin_groups(g, out.cap_bins) + PUSH_STAGE_WGS);
     s->diag_nb = (int)grid;
     hipEvent_t evk;
     timer_begin(w, PIC_T_PUSH_KERNEL, &evk);
-    if (s->prec == PIC_MIXED)
-        k_push<PIC_MIXED><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
-    else
-        k_push<PIC_FP64><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+    // the two-cell-rim push, then the wide-rim one for the tiles the plan flagged (each
+    // returns at once on the other's workgroups)
+    if (s->prec == PIC_MIXED) {
+        k_push<PIC_MIXED, HALO><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+        k_push<PIC_MIXED, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+    } else {
+        k_push<PIC_FP64, HALO><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+        k_push<PIC_FP64, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+    }
     PIC_LAUNCH_CHECK();
     timer_end(w, PIC_T_PUSH_KERNEL, evk);
     timer_end(w, PIC_T_PUSH, ev);

@@ -30,6 +30,11 @@ constexpr int TILE = 1 << TILE_SHIFT;
 constexpr int TN = TILE + 1;              // E-tile nodes per edge
 constexpr int TNODES = TN * TN * TN;      // 729
 constexpr int HALO = 2;
+// Tiles whose particles left the two-cell rim often in the last push (ions the sphere
+// accelerates move several cells a step) are pushed by a second instantiation with a
+// HALO_WIDE-cell rim; the part of its window beyond HALO cells goes out by global atomics
+// once per node.
+constexpr int HALO_WIDE = 4;
 constexpr int WN = TILE + 1 + 2 * HALO;   // window nodes per edge
 constexpr int WNODES = WN * WN * WN;      // 2197
 constexpr int PUSH_THREADS = 256;
@@ -218,6 +223,8 @@ struct pic_species {
     int ave_samples = 0;
     void *win = nullptr;          // per-tile deposit windows (float or double)
     uint8_t *win_used = nullptr;
+    uint32_t *far_tile = nullptr;  // per tile: far deposits of the last push, then [ntiles..) the
+                                   // plan's wide-rim flags
     void *plan_mem = nullptr;     // scratch of the segment plan (per-tile counts, block sums)
     uint64_t *chunk_map = nullptr;  // push workgroup -> (tile, chunk), written by the plan
     long chunk_cap = 0;           // entries of chunk_map

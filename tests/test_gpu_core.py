@@ -220,6 +220,44 @@ def test_push_absorb_sphere_mixed_matches_oracle():
     assert rel_l2(r["den"], r["den_ref"]) < 1e-4
 
 
+# The wide-rim push (a four-cell window rim for the tiles whose particles left the two-cell
+# one in the last push): forced for every tile, and chosen per tile from the far counts.  At
+# 3e7 m/s and 1e-9 s a particle moves up to 3 cells a step, so many land outside the two-cell
+# rim and inside the four-cell one.
+@pytest.mark.parametrize("mode", ["all", "auto"])
+def test_push_wide_rim_fp64_matches_oracle(mode, monkeypatch):
+    if mode == "all":
+        monkeypatch.setenv("PIC_PUSH_WIDE", "all")
+    r = _push_case(P.FP64, P.REFLECT, 10, 1e-9, 3e7)
+    assert np.array_equal(r["gi"], r["ri"])
+    np.testing.assert_allclose(r["gp"], r["rp"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(r["gv"], r["rv"], rtol=1e-14, atol=1e-9)
+    assert rel_l2(r["den"], r["den_ref"]) < 1e-13
+    d = O.species_diag(r["rv"], r["rw"], O.ME)
+    np.testing.assert_allclose(r["diag"], d, rtol=1e-12, atol=1e-40)
+
+
+@pytest.mark.parametrize("mode", ["all", "auto"])
+def test_push_wide_rim_mixed_matches_oracle(mode, monkeypatch):
+    if mode == "all":
+        monkeypatch.setenv("PIC_PUSH_WIDE", "all")
+    r = _push_case(P.MIXED, P.REFLECT, 10, 1e-9, 3e7)
+    assert np.array_equal(r["gi"], r["ri"])
+    dcell = np.abs(r["gp"] - r["rp"]) / r["g"].dh
+    assert dcell.max() < 2e-4, dcell.max()
+    assert rel_l2(r["den"], r["den_ref"]) < 1e-5
+
+
+def test_push_wide_rim_split_tile_matches_oracle(monkeypatch):
+    # seven workgroups share the first tile; with the wide rim every one of them adds its
+    # window beyond the slot to the far-deposit array
+    monkeypatch.setenv("PIC_PUSH_WIDE", "all")
+    r = _push_case(P.FP64, P.REFLECT, 6, 1e-9, 3e6, **SPLIT)
+    assert len(r["gi"]) == SPLIT["n"] and np.array_equal(r["gi"], r["ri"])
+    np.testing.assert_allclose(r["gp"], r["rp"], rtol=0, atol=1e-15)
+    assert rel_l2(r["den"], r["den_ref"]) < 1e-13
+
+
 def test_sor_linear_converges_to_oracle_criterion():
     w, g = make_world()
     rng = np.random.default_rng(2)
