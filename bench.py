@@ -352,7 +352,8 @@ def main():
             f"push {fp64['push_avg_ms']:.3f} ms/launch")
 
     # roofline of the dominant kernel (the fused push): algorithmic bytes per launch over
-    # the kernel's average HIP-event duration (events around the k_push launch alone)
+    # the kernel's average HIP-event duration (events around the k_push launches alone: the
+    # two-cell-rim instantiation and the wide-rim one for the tiles the plan flagged)
     steps = args.steps
     np_avg = mixed["particle_steps"] / steps
     bytes_launch = np_avg * BYTES_PER_PARTICLE_STEP + mixed["nn"] * BYTES_PER_NODE_E
@@ -368,7 +369,7 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "kernel": "k_push<PIC_MIXED>", "bytes_per_launch": bytes_launch,
+                "kernel": "k_push<PIC_MIXED, HALO> + k_push<PIC_MIXED, HALO_WIDE>", "bytes_per_launch": bytes_launch,
                 "kernel_ms": mixed["push_kernel_avg_ms"],
                 "copy_kernel_gbps": copy_gbps}
 

@@ -637,8 +637,8 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_starts(Grid g, PlanTmp T,
 }
 
 // push workgroups per tile: one per PUSH_CHUNK of its input particles, at least one
-// and which instantiation pushes each tile: the wide-rim one when at least 32 of its
-// particles, and 1/32 of them, deposited outside the window in the last push (force: 1 all
+// and which instantiation pushes each tile: the wide-rim one when at least 16 of its
+// particles, and 1/128 of them, deposited outside the window in the last push (force: 1 all
 // tiles, -1 none); the far counts restart from 0
 __global__ void __launch_bounds__(PLAN_THREADS)
     k_plan_chunks(Grid g, PlanTmp T, uint32_t *far, uint32_t *wide, int force) {
@@ -650,7 +650,7 @@ __global__ void __launch_bounds__(PLAN_THREADS)
         T.nchunk[t] = c;
         uint32_t f = far[t];
         far[t] = 0;
-        wide[t] = force > 0 ? 1u : force < 0 ? 0u : (f >= 32u && (unsigned long long)f * 32u >= T.need[t]) ? 1u : 0u;
+        wide[t] = force > 0 ? 1u : force < 0 ? 0u : (f >= 16u && (unsigned long long)f * 128u >= T.need[t]) ? 1u : 0u;
     }
     unsigned long long a = block_sum_u64(c, sh);
     if (threadIdx.x == 0) T.blk2[blockIdx.x] = a;

@@ -33,8 +33,9 @@ constexpr int HALO = 2;
 // Tiles whose particles left the two-cell rim often in the last push (ions the sphere
 // accelerates move several cells a step) are pushed by a second instantiation with a
 // HALO_WIDE-cell rim; the part of its window beyond HALO cells goes out by global atomics
-// once per node.
-constexpr int HALO_WIDE = 4;
+// once per node.  Three cells measured faster than four late in the sphere run (17.9 against
+// 18.7 ms per push at steps 81-100; five: 19.6): 4 workgroups per CU against 3.
+constexpr int HALO_WIDE = 3;
 constexpr int WN = TILE + 1 + 2 * HALO;   // window nodes per edge
 constexpr int WNODES = WN * WN * WN;      // 2197
 constexpr int PUSH_THREADS = 256;

@@ -220,10 +220,10 @@ def test_push_absorb_sphere_mixed_matches_oracle():
     assert rel_l2(r["den"], r["den_ref"]) < 1e-4
 
 
-# The wide-rim push (a four-cell window rim for the tiles whose particles left the two-cell
+# The wide-rim push (a three-cell window rim for the tiles whose particles left the two-cell
 # one in the last push): forced for every tile, and chosen per tile from the far counts.  At
 # 3e7 m/s and 1e-9 s a particle moves up to 3 cells a step, so many land outside the two-cell
-# rim and inside the four-cell one.
+# rim, and some outside the three-cell one too.
 @pytest.mark.parametrize("mode", ["all", "auto"])
 def test_push_wide_rim_fp64_matches_oracle(mode, monkeypatch):
     if mode == "all":
