@@ -656,9 +656,12 @@ __global__ void __launch_bounds__(PLAN_THREADS)
     if (threadIdx.x == 0) T.blk2[blockIdx.x] = a;
 }
 
-// map[w] = tile | chunk << 32 for push workgroup w, tiles in order; entries past the map's
-// capacity (only when the particle count exceeds the bins, which the plan flags) are dropped
-__global__ void __launch_bounds__(PLAN_THREADS) k_plan_map(Grid g, PlanTmp T, uint64_t *map, long cap) {
+// map[w] = tile | chunk << 32 | wide << 63 for push workgroup w, tiles in order (wide: the
+// tile's flag from k_plan_chunks, read by each push workgroup with its entry); entries past
+// the map's capacity (only when the particle count exceeds the bins, which the plan flags)
+// are dropped
+__global__ void __launch_bounds__(PLAN_THREADS)
+    k_plan_map(Grid g, PlanTmp T, const uint32_t *wide, uint64_t *map, long cap) {
     __shared__ unsigned long long s_scan[PLAN_THREADS];
     int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
     unsigned long long c = t < g.ntiles ? T.nchunk[t] : 0;
@@ -672,8 +675,9 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_map(Grid g, PlanTmp T, ui
     }
     if (t < g.ntiles) {
         unsigned long long w0 = T.blk2[blockIdx.x] + s_scan[threadIdx.x] - c;
+        const uint64_t wb = wide[t] ? (1ull << 63) : 0ull;
         for (unsigned long long k = 0; k < c && w0 + k < (unsigned long long)cap; k++)
-            map[w0 + k] = (uint64_t)t | ((uint64_t)k << 32);
+            map[w0 + k] = (uint64_t)t | ((uint64_t)k << 32) | wb;
     }
 }
 
@@ -681,7 +685,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_map(Grid g, PlanTmp T, ui
 // the fused push: gather (LDS E tile) + leapfrog + walls + CIC deposit (LDS window) + rebin
 // ---------------------------------------------------------------------------------------
 struct PushOut {
-    const uint64_t *chunk_map;           // workgroup -> tile | chunk << 32
+    const uint64_t *chunk_map;           // workgroup -> tile | chunk << 32 | wide-rim flag << 63
     const unsigned long long *total_groups;  // workgroups the map holds
     float *win32;
     double *win64;
@@ -690,7 +694,6 @@ struct PushOut {
     double *diag_part;  // [NDIAG_SLOTS][gridDim.x] per-block partials, reduced on demand
     uint32_t *err;
     uint32_t *far;          // per tile: particles deposited outside the window this push
-    const uint32_t *wide;   // per tile: pushed by the wide-rim instantiation
 };
 
 // slot of an arrival in the inflow segment of tile dtile, -1 when the segment is full
@@ -773,8 +776,8 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         }
         const uint64_t tc = O.chunk_map[wg];
         const int t = (int)(uint32_t)tc;
-        const uint32_t chunk = (uint32_t)(tc >> 32);
-        if ((O.wide[t] != 0u) != WIDE) return;  // the other instantiation's tile
+        const uint32_t chunk = (uint32_t)(tc >> 32) & 0x7fffffffu;
+        if (((tc >> 63) != 0) != WIDE) return;  // the other instantiation's tile
         BinSeg bs = bin_of(in, t);
         const uint32_t cnt_tile = bs.c0 + bs.c1;
         const bool split = cnt_tile > (uint32_t)PUSH_CHUNK;  // several workgroups share tile t
@@ -1597,7 +1600,7 @@ static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist,
         const int force = !fw ? 0 : !strcmp(fw, "all") ? 1 : !strcmp(fw, "off") ? -1 : 0;
         k_plan_chunks<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->far_tile, s->far_tile + g.ntiles, force);
         k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk2, nblk, T.total);
-        k_plan_map<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->chunk_map, s->chunk_cap);
+        k_plan_map<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->far_tile + g.ntiles, s->chunk_map, s->chunk_cap);
     }
     PIC_LAUNCH_CHECK();
     return PIC_OK;
@@ -2023,7 +2026,6 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     O.diag_part = s->diag_part;
     O.err = err_flag(s);
     O.far = s->far_tile;
-    O.wide = s->far_tile + g.ntiles;
     // workgroups for the tiles (the plan's map decides which run) + the staging workgroups
     unsigned grid = (unsigned)(push_bin_groups(g, out.cap_bins) + PUSH_STAGE_WGS);
     s->diag_nb = (int)grid;
