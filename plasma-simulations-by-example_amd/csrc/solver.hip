@@ -564,11 +564,11 @@ __global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid
 // reduced diagonal jr = A(u,u) - P(u) + sum of Neumann-neighbour coefficients, and a mask:
 // bits 0-5 regular neighbours in direction d, bit 6 the node itself is regular.
 // CgStart (nullable): the start of a correction solve from y = 0, written here instead of
-// by a pass of its own: g = J 0 - F = -F (FP64 and FP32), d = 0, y = 0
+// by a pass of its own: g = J 0 - F = -F (FP64 and FP32).  The first CG pass takes d = 0
+// and y = 0 without reading them (k_mgpcg_a<true>).
 struct CgStart {
     double *g;
     float *g32;
-    double *d0, *y;
 };
 
 __global__ void __launch_bounds__(256)
@@ -577,10 +577,6 @@ __global__ void __launch_bounds__(256)
     long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     if (u >= g.nn) return;
-    if (st.g) {
-        st.d0[u] = 0;
-        st.y[u] = 0;
-    }
     int i, j, k;
     node_ijk(g, u, i, j, k);
     int t = node_type(g, oid, u, i, j, k);
@@ -687,6 +683,8 @@ __global__ void __launch_bounds__(256)
 //   s = V(g), with the partials of g.s from its last sweep:  mu = a'/a
 // Scalars: red[0] a, red[1] b, red[4] RMS(g), red[6] mu, red[7] lam = a/b.
 
+// FIRST: the first pass of a solve from y = 0, d = 0 (mu = lam = 0), neither of them read
+template <bool FIRST>
 __global__ void __launch_bounds__(256)
     k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const float *sv, const double *d_old,
               double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
@@ -706,13 +704,14 @@ __global__ void __launch_bounds__(256)
         if (in) {
             long sj = g.ni, sk = (long)g.ni * g.nj;
             uint8_t m = mask[u];
-            double jru = jr[u], dou = d_old[u], yu = y[u];
-            double n0 = mu * d_old[u - sk] - (double)sv[u - sk];
-            double n1 = mu * d_old[u - sj] - (double)sv[u - sj];
-            double n2 = mu * d_old[u - 1] - (double)sv[u - 1];
-            double n3 = mu * d_old[u + 1] - (double)sv[u + 1];
-            double n4 = mu * d_old[u + sj] - (double)sv[u + sj];
-            double n5 = mu * d_old[u + sk] - (double)sv[u + sk];
+            auto dold = [&](long v) { return FIRST ? 0.0 : d_old[v]; };
+            double jru = jr[u], dou = dold(u), yu = FIRST ? 0.0 : y[u];
+            double n0 = mu * dold(u - sk) - (double)sv[u - sk];
+            double n1 = mu * dold(u - sj) - (double)sv[u - sj];
+            double n2 = mu * dold(u - 1) - (double)sv[u - 1];
+            double n3 = mu * dold(u + 1) - (double)sv[u + 1];
+            double n4 = mu * dold(u + sj) - (double)sv[u + sj];
+            double n5 = mu * dold(u + sk) - (double)sv[u + sk];
             if (m & 64) {
                 y[u] = yu + lam * dou;
                 dc = mu * dou - (double)sv[u];
@@ -1168,8 +1167,12 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     for (int it0 = 0; it0 < s->max_it; it0 += CHUNK) {
         int it1 = std::min(s->max_it, it0 + CHUNK);
         for (int it = it0; it < it1; it++) {
-            k_mgpcg_a<<<nb8, 256, 0, st>>>(Q, s->mask, s->P, sv, dbuf[it & 1], dbuf[(it + 1) & 1], s->y, s->z,
-                                         w->partials, w->scal);
+            if (it == 0)
+                k_mgpcg_a<true><<<nb8, 256, 0, st>>>(Q, s->mask, s->P, sv, dbuf[0], dbuf[1], s->y, s->z, w->partials,
+                                                     w->scal);
+            else
+                k_mgpcg_a<false><<<nb8, 256, 0, st>>>(Q, s->mask, s->P, sv, dbuf[it & 1], dbuf[(it + 1) & 1], s->y,
+                                                      s->z, w->partials, w->scal);
             k_fold<<<FOLD_GRID, 256, 0, st>>>(w->partials, nb8, folded, w->scal);
             k_mgpcg_scalar<<<1, 1024, 0, st>>>(folded, (int)FOLD_GRID, 1, nu, Q.tol, it, w->scal);
             k_mgpcg_b<<<G, 256, 0, st>>>(nn, s->g, s->g32, s->z, w->partials, w->scal);
@@ -1188,7 +1191,10 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
         }
     }
     int done_its = w->scal_host->iters;
-    k_pcg_finish<<<nb8, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    if (done_its > 0)
+        k_pcg_finish<<<nb8, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    else  // converged at the start (or no iteration allowed): the correction is 0
+        k_fill<<<nbk(nn), 256, 0, st>>>(nn, s->y, 0.0);
     PIC_LAUNCH_CHECK();
     *its += done_its;
     *converged = conv;
@@ -1232,8 +1238,8 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
         // residual F, where the reference's warm start from the previous correction (kept
         // by the PCG path) begins about as far from the new correction as that one is.
         // With the multigrid preconditioner, k_nrs_F also writes that start.
-        CgStart cs{nullptr, nullptr, nullptr, nullptr};
-        if (use_mg) cs = CgStart{s->g, s->g32, s->d, s->y};
+        CgStart cs{nullptr, nullptr};
+        if (use_mg) cs = CgStart{s->g, s->g32};
         k_nrs_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz, cs);
         if (!use_mg && it > 0) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
         PIC_LAUNCH_CHECK();
