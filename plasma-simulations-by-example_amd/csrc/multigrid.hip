@@ -888,8 +888,8 @@ int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const doubl
     for (size_t l = 0; l < H->L.size(); l++) {
         MgLevel &c = H->L[l];
         c.Q = level_params(Q0, c.g);
-        if (l == 0) {
-            k_mg_tof<<<nbk8(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.idiag);
+        if (l == 0) {  // (jr0 null: the Newton residual kernel wrote level 0's 1/diag itself)
+            if (jr0) k_mg_tof<<<nbk8(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.idiag);
             continue;
         }
         const MgLevel &f = H->L[l - 1];
@@ -1027,5 +1027,8 @@ int mg_vcycle(MgHierarchy *H, const double *b0, const float *b0f, float **x0, do
 int mg_dot_partials(const MgHierarchy *H) { return (int)smooth_blocks(H->L[0].g); }
 
 int mg_levels(const MgHierarchy *H) { return H ? (int)H->L.size() : 0; }
+
+// level 0's FP32 reciprocal diagonal (written by the Newton residual kernel)
+float *mg_level0_idiag(MgHierarchy *H) { return H->L[0].idiag; }
 
 }  // namespace pic

@@ -565,10 +565,12 @@ __global__ void __launch_bounds__(256) k_neumann_copy(Grid g, const uint8_t *oid
 // bits 0-5 regular neighbours in direction d, bit 6 the node itself is regular.
 // CgStart (nullable): the start of a correction solve from y = 0, written here instead of
 // by a pass of its own: g = J 0 - F = -F (FP64 and FP32).  The first CG pass takes d = 0
-// and y = 0 without reading them (k_mgpcg_a<true>).
+// and y = 0 without reading them (k_mgpcg_a<true>).  idiag32: the multigrid's level-0
+// reciprocal diagonal in FP32, (float)(1 / jr).
 struct CgStart {
     double *g;
     float *g32;
+    float *idiag32;
 };
 
 __global__ void __launch_bounds__(256)
@@ -588,6 +590,7 @@ __global__ void __launch_bounds__(256)
         if (st.g) {
             st.g[u] = 0;
             st.g32[u] = 0.0f;
+            st.idiag32[u] = 1.0f;
         }
         return;
     }
@@ -612,6 +615,7 @@ __global__ void __launch_bounds__(256)
     if (st.g) {
         st.g[u] = -f;
         st.g32[u] = (float)(-f);
+        st.idiag32[u] = (float)(1.0 / diag);
     }
 }
 
@@ -1137,6 +1141,7 @@ int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const doubl
 int mg_vcycle(MgHierarchy *H, const double *b0, const float *b0f, float **x0, double *dot_partials, hipStream_t st,
               const DevScalars *sc);
 int mg_dot_partials(const MgHierarchy *H);
+float *mg_level0_idiag(MgHierarchy *H);
 void mg_destroy(MgHierarchy *H);
 
 // solvePCGLinear on the reduced system with one V-cycle as the preconditioner
@@ -1238,14 +1243,14 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
         // residual F, where the reference's warm start from the previous correction (kept
         // by the PCG path) begins about as far from the new correction as that one is.
         // With the multigrid preconditioner, k_nrs_F also writes that start.
-        CgStart cs{nullptr, nullptr};
-        if (use_mg) cs = CgStart{s->g, s->g32};
+        CgStart cs{nullptr, nullptr, nullptr};
+        if (use_mg) cs = CgStart{s->g, s->g32, mg_level0_idiag(s->mg)};
         k_nrs_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz, cs);
         if (!use_mg && it > 0) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
         PIC_LAUNCH_CHECK();
         bool pc = false;
         if (use_mg) {
-            rc = mg_setup(s->mg, Q, s->P, s->boltz, w->geom_stamp, w->stream);
+            rc = mg_setup(s->mg, Q, nullptr, s->boltz, w->geom_stamp, w->stream);
             if (rc) return rc;
             rc = mgpcg_linear(s, Q, &pc, &pcg_its);
         } else {
