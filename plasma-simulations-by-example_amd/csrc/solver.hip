@@ -817,20 +817,25 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
-// y on face nodes := y at their chain target, 0 on Dirichlet nodes; x -= y; partials of y.y
+// y on face nodes := y at their chain target, 0 on Dirichlet nodes; x -= y; partials of y.y.
+// d_pend (nullable): the CG's last deferred update y += lam d is applied here, on the values
+// read, instead of by k_pcg_finish (lam = sc->red[7]; the same operations, so the same y)
 __global__ void __launch_bounds__(256)
-    k_nrs_update(PcgParams Q, const uint8_t *oid, double *x, double *y, double *partials) {
+    k_nrs_update(PcgParams Q, const uint8_t *oid, double *x, const double *y, const double *d_pend,
+                 const DevScalars *sc, double *partials) {
     __shared__ double sh[4];
     long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
+    const double lam = d_pend ? sc->red[7] : 0.0;
+    auto yv = [&](long v) { return d_pend ? y[v] + lam * d_pend[v] : y[v]; };
     double y2 = 0;
     if (u < g.nn) {
         int i, j, k;
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
-        double yy = y[u];
-        if (t == NT_DIR) yy = 0;
-        else if (t == NT_NEU) yy = y[chain_target(g, oid, u, i, j, k)];
+        double yy = 0;
+        if (t == NT_REG) yy = yv(u);
+        else if (t == NT_NEU) yy = yv(chain_target(g, oid, u, i, j, k));
         x[u] = x[u] - yy;
         y2 = yy * yy;
     }
@@ -1145,7 +1150,9 @@ float *mg_level0_idiag(MgHierarchy *H);
 void mg_destroy(MgHierarchy *H);
 
 // solvePCGLinear on the reduced system with one V-cycle as the preconditioner
-static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *its) {
+// *d_pend: when converged, the direction whose update y += lam d the caller's k_nrs_update
+// applies (null when y is final here)
+static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *its, const double **d_pend) {
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
@@ -1196,7 +1203,10 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
         }
     }
     int done_its = w->scal_host->iters;
-    if (done_its > 0)
+    *d_pend = nullptr;
+    if (done_its > 0 && conv)
+        *d_pend = dbuf[done_its & 1];
+    else if (done_its > 0)  // the Gauss-Seidel fallback continues from y
         k_pcg_finish<<<nb8, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
     else  // converged at the start (or no iteration allowed): the correction is 0
         k_fill<<<nbk(nn), 256, 0, st>>>(nn, s->y, 0.0);
@@ -1249,10 +1259,11 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
         if (!use_mg && it > 0) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
         PIC_LAUNCH_CHECK();
         bool pc = false;
+        const double *d_pend = nullptr;
         if (use_mg) {
             rc = mg_setup(s->mg, Q, nullptr, s->boltz, w->geom_stamp, w->stream);
             if (rc) return rc;
-            rc = mgpcg_linear(s, Q, &pc, &pcg_its);
+            rc = mgpcg_linear(s, Q, &pc, &pcg_its, &d_pend);
         } else {
             rc = pcgs_linear(s, Q, &pc, &pcg_its);
         }
@@ -1261,7 +1272,7 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
             rc = gsls_linear(s, Q);
             if (rc) return rc;
         }
-        k_nrs_update<<<nb8, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
+        k_nrs_update<<<nb8, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, d_pend, w->scal, w->partials);
         k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb8, &w->scal->red[5]);
         PIC_LAUNCH_CHECK();
         rc = read_scalars(w);
