@@ -116,12 +116,6 @@ __global__ void __launch_bounds__(256)
     P_c[u] = (float)P_f[f];
 }
 
-// level-0 reciprocal diagonal in FP32
-__global__ void __launch_bounds__(256) k_mg_tof(long n, const double *a, float *b) {
-    long u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u < n) b[u] = (float)(1.0 / a[u]);
-}
-
 // mask and reduced diagonal of a coarse level (the operator part of k_nrs_F)
 __global__ void __launch_bounds__(256)
     k_mg_ops(PcgParams Q, const uint8_t *oid, const float *P, uint8_t *mask, float *idiag) {
@@ -879,19 +873,16 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
     return PIC_OK;
 }
 
-// Level operators for the current Newton step: level 0's mask, jr and P come fresh from
-// k_nrs_F.  The Laplacian parts of the Galerkin operators depend on the geometry only and
-// are rebuilt when its stamp changes; their diagonals take each step's injected P.
-int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, unsigned geom_stamp,
-             hipStream_t st) {
+// Level operators for the current Newton step: level 0's mask, P and FP32 reciprocal
+// diagonal come fresh from k_nrs_F.  The Laplacian parts of the Galerkin operators depend on
+// the geometry only and are rebuilt when its stamp changes; their diagonals take each step's
+// injected P.
+int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *boltz0, unsigned geom_stamp, hipStream_t st) {
     const bool rap = H->rap_stamp != geom_stamp;
     for (size_t l = 0; l < H->L.size(); l++) {
         MgLevel &c = H->L[l];
         c.Q = level_params(Q0, c.g);
-        if (l == 0) {  // (jr0 null: the Newton residual kernel wrote level 0's 1/diag itself)
-            if (jr0) k_mg_tof<<<nbk8(c.g.nn), 256, 0, st>>>(c.g.nn, jr0, c.idiag);
-            continue;
-        }
+        if (l == 0) continue;
         const MgLevel &f = H->L[l - 1];
         if (l == 1)
             k_mg_inject<double><<<nbk8(c.g.nn), 256, 0, st>>>(f.g, c.g, f.oid, boltz0, c.oid, c.boltz);

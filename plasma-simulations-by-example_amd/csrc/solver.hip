@@ -1141,8 +1141,7 @@ static int gsls_linear(pic_solver *s, const PcgParams &Q) {
 
 // Newton iteration of solveNRPCG (:225-296) on the Neumann-reduced system
 int mg_build(pic_solver *s, MgHierarchy **out);
-int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *jr0, const double *boltz0, unsigned geom_stamp,
-             hipStream_t st);
+int mg_setup(MgHierarchy *H, const PcgParams &Q0, const double *boltz0, unsigned geom_stamp, hipStream_t st);
 int mg_vcycle(MgHierarchy *H, const double *b0, const float *b0f, float **x0, double *dot_partials, hipStream_t st,
               const DevScalars *sc);
 int mg_dot_partials(const MgHierarchy *H);
@@ -1261,7 +1260,7 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
         bool pc = false;
         const double *d_pend = nullptr;
         if (use_mg) {
-            rc = mg_setup(s->mg, Q, nullptr, s->boltz, w->geom_stamp, w->stream);
+            rc = mg_setup(s->mg, Q, s->boltz, w->geom_stamp, w->stream);
             if (rc) return rc;
             rc = mgpcg_linear(s, Q, &pc, &pcg_its, &d_pend);
         } else {
