@@ -29,8 +29,8 @@
 extern "C" {
 #endif
 
-/* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL) */
-#define PICAMD_ABI_VERSION 2
+/* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL); 3: pic_solver_get_info */
+#define PICAMD_ABI_VERSION 3
 
 /* status codes */
 #define PIC_OK 0
@@ -147,7 +147,9 @@ int pic_species_get_particles(pic_species *s, double *pos, double *vel, double *
 int pic_species_get_np(pic_species *s, long *np);
 /* Species::advance: fused leapfrog push + CIC gather (+ CIC deposit of the new positions,
  * consumed by computeNumberDensity) with reflecting (ch2/Species.cpp:7-38) or absorbing
- * walls/sphere and compaction (ch3/ver2/Species.cpp:7-48). */
+ * walls/sphere and compaction (ch3/ver2/Species.cpp:7-48).  A push that had to drop
+ * particles (its staging list full) is reported one call later, without a stream sync: the
+ * next advance, computeNumberDensity, inject or getNp returns PIC_ERR_CAPACITY. */
 int pic_species_advance(pic_species *s, int bc, double dt);
 /* Species::computeNumberDensity (ch3/ver2/Species.cpp:51-62): den = scatter(mpw)/node_vol.
  * Reuses the deposit fused into advance when nothing changed since; deposits afresh
@@ -189,6 +191,12 @@ int pic_solver_set_reference_values(pic_solver *s, double phi0, double Te0, doub
  * the last convergence norm, iters (nullable) the iterations (PCG: total inner PCG
  * iterations over the Newton steps). */
 int pic_solver_solve(pic_solver *s, double *residual, int *iters);
+/* What the last solve() did, for callers that check which path ran: info[0] Newton
+ * iterations (NR solvers; 0 otherwise), info[1] the iterations solve() reported (SOR sweeps,
+ * or inner PCG iterations over all Newton steps), info[2] how many Newton steps fell back
+ * from PCG to solveGSLinear (ch3/ver2/PotentialSolver.cpp:270-271), info[3] 1 when it
+ * converged. */
+int pic_solver_get_info(pic_solver *s, int info[4]);
 /* solveQN() with the solver's reference values (ch3/ver2/PotentialSolver.cpp:204-222). */
 int pic_solver_solve_qn(pic_solver *s);
 /* computeEF() (ch3/ver2/PotentialSolver.cpp:465-504). */

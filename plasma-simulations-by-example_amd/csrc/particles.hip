@@ -1463,6 +1463,23 @@ __global__ void k_fill_f64(long n, double *p, double v) {
 
 __global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
 
+// Snapshot of a push, in stream order right after it: the live count (sum of the push's
+// per-workgroup count partials), the staging count, and the capacity flags raised since the
+// last snapshot (taken and cleared in one atomic, so none is lost to a later push).
+__global__ void __launch_bounds__(256)
+    k_push_status(const double *cnt_part, int nb, const uint32_t *stage_count, long cap_stage, SpStatus *st) {
+    __shared__ double sh[4];
+    double a = 0;
+    for (int i = threadIdx.x; i < nb; i += 256) a += cnt_part[i];
+    a = block_sum(a, sh);
+    if (threadIdx.x == 0) {
+        st->np = (long long)(a + 0.5);
+        const uint32_t sc = *stage_count;
+        st->stage = (long)sc < cap_stage ? sc : (uint32_t)cap_stage;
+        st->err_snap = atomicExch(&st->err, 0u);
+    }
+}
+
 inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
 
 // =========================================================================================
@@ -1611,10 +1628,40 @@ static int launch_plan_push(pic_species *s, uint32_t *err) {
     return launch_plan(s, view_of(in), nullptr, 0, out, err);
 }
 
-// error flags of the last device operations live in DevScalars::counter slot of the world
-static uint32_t *err_flag(pic_species *s) { return &s->w->scal->counter; }
+// capacity flags of this species' device operations (1: bins short, 2: particle dropped,
+// 4: sort overflow)
+static uint32_t *err_flag(pic_species *s) { return &s->st_dev->err; }
+
+// Reads the snapshot of the last push once its copy has landed (wait: blocks on its event
+// only, not on the stream).  Tightens np_upper to the pushed count plus what was appended
+// since, and learns the staging count; PIC_ERR_CAPACITY when that push dropped particles.
+static int poll_status(pic_species *s, bool wait) {
+    if (!s->st_pending) return PIC_OK;
+    if (wait) {
+        PIC_HIP(hipEventSynchronize(s->st_ev));
+    } else {
+        hipError_t q = hipEventQuery(s->st_ev);
+        if (q == hipErrorNotReady) return PIC_OK;
+        if (q != hipSuccess) return hip_fail(q, "hipEventQuery", __FILE__, __LINE__);
+    }
+    s->st_pending = false;
+    const SpStatus &st = *s->st_host;
+    s->np_upper = (long)st.np + s->added_since_push;
+    if (!s->stage_known) {
+        s->stage_host = (long)st.stage + s->added_since_push;
+        s->stage_known = true;
+    }
+    if (st.err_snap & 6u) {  // 2: staging full in a push or injection, 4: sort overflow
+        s->np_exact = false;  // the snapshot counted the dropped particles
+        set_error("the particle staging list overflowed in advance(): particles were dropped (capacity)");
+        return PIC_ERR_CAPACITY;
+    }
+    return PIC_OK;
+}
 
 static int check_err(pic_species *s) {
+    int rc = poll_status(s, true);
+    if (rc) return rc;
     uint32_t e = 0;
     PIC_HIP(hipMemcpyAsync(&s->w->scal_host->counter, err_flag(s), 4, hipMemcpyDeviceToHost, s->w->stream));
     PIC_HIP(hipStreamSynchronize(s->w->stream));
@@ -1685,6 +1732,8 @@ static int sort_launch(pic_species *s) {
     s->cur = 1 - s->cur;
     s->sorted = true;
     s->stage_added = 0;
+    s->stage_host = 0;  // everything is in the bins now
+    s->stage_known = true;
     return PIC_OK;
 }
 
@@ -1750,8 +1799,9 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
             goto fail;                                         \
         }                                                      \
     } while (0)
-    TRY(hipMalloc(&s->np_dev, sizeof(long)));
-    TRY(hipHostMalloc(&s->np_host, sizeof(long)));
+    TRY(hipMalloc(&s->st_dev, sizeof(SpStatus)));
+    TRY(hipHostMalloc(&s->st_host, sizeof(SpStatus)));
+    TRY(hipEventCreateWithFlags(&s->st_ev, hipEventDisableTiming));
     TRY(hipMalloc(&s->den, sizeof(double) * g.nn));
     TRY(hipMalloc(&s->den_extra, sizeof(double) * g.nn));
     TRY(hipMalloc(&s->den_ave, sizeof(double) * g.nn));
@@ -1762,7 +1812,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMalloc(&s->diag_dev, sizeof(double) * 8));
     if ((rc = ensure_push_scratch(s))) goto fail;
     TRY(hipHostMalloc(&s->diag_host, sizeof(double) * 8));
-    TRY(hipMemsetAsync(s->np_dev, 0, sizeof(long), w->stream));
+    TRY(hipMemsetAsync(s->st_dev, 0, sizeof(SpStatus), w->stream));
     TRY(hipMemsetAsync(s->den, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->den_extra, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->far_tile, 0, sizeof(uint32_t) * 2 * (size_t)g.ntiles, w->stream));
@@ -1772,7 +1822,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMemsetAsync(s->diag_dev, 0, sizeof(double) * 8, w->stream));
     TRY(hipStreamSynchronize(w->stream));
 #undef TRY
-    *s->np_host = 0;
+    *s->st_host = SpStatus{};
     s->den_state = 2;  // no particles: den = 0 is current
     s->sorted = true;
     *out = s;
@@ -1787,8 +1837,9 @@ int pic_species_destroy(pic_species *s) {
     if (s->w && s->w->stream) hipStreamSynchronize(s->w->stream);
     free_buf(s->buf[0]);
     free_buf(s->buf[1]);
-    hipFree(s->np_dev);
-    if (s->np_host) hipHostFree(s->np_host);
+    hipFree(s->st_dev);
+    if (s->st_host) hipHostFree(s->st_host);
+    if (s->st_ev) hipEventDestroy(s->st_ev);
     hipFree(s->den);
     hipFree(s->den_extra);
     hipFree(s->den_ave);
@@ -1847,8 +1898,12 @@ static int append_host(pic_species *s, long n, const double *pos, const double *
     PIC_HIP(hipStreamSynchronize(w->stream));
     PIC_HIP(hipFree(d));
     if (!ids) s->next_id += (uint32_t)n;
+    // count_particles synchronised the stream: any push snapshot has landed and is stale now
+    s->st_pending = false;
     s->np_upper = inbins + stage + n;
-    *s->np_host = s->np_upper;
+    s->added_since_push = 0;
+    s->stage_host = stage + n;
+    s->stage_known = true;
     s->stage_added += n;
     s->sorted = false;
     s->den_state = 0;
@@ -1897,7 +1952,10 @@ int pic_species_set_particles(pic_species *s, long n, const double *pos, const d
     }
     s->next_id = 0;
     s->np_upper = 0;
-    *s->np_host = 0;
+    s->st_pending = false;
+    s->added_since_push = 0;
+    s->stage_host = 0;
+    s->stage_known = true;
     if (n == 0) {
         s->den_state = 0;
         s->diag_valid = false;
@@ -1910,6 +1968,14 @@ int pic_species_set_particles(pic_species *s, long n, const double *pos, const d
 
 int pic_species_get_np(pic_species *s, long *np) {
     PIC_CHECK(s && np, PIC_ERR_ARG, "null argument");
+    // the last push's snapshot (waited for on its event) plus what was appended since is
+    // exact unless that push dropped particles, which poll_status reports
+    int rc0 = poll_status(s, true);
+    if (rc0) return rc0;
+    if (s->np_exact) {
+        *np = s->np_upper;
+        return PIC_OK;
+    }
     std::vector<uint32_t> cnt;
     uint32_t stage = 0;
     int rc = count_particles(s, cnt, stage);
@@ -1918,6 +1984,8 @@ int pic_species_get_np(pic_species *s, long *np) {
     for (auto c : cnt) tot += c;
     *np = tot;
     s->np_upper = tot;
+    s->np_exact = true;
+    s->added_since_push = 0;
     return check_err(s);
 }
 
@@ -1975,7 +2043,10 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     PIC_CHECK(bc == PIC_BC_REFLECT || bc == PIC_BC_ABSORB, PIC_ERR_ARG, "bad boundary");
     pic_world *w = s->w;
     const Grid &g = w->g;
-    int rc;
+    // the previous push's snapshot: particles it dropped are reported here, one call later
+    // (the wait is on that push's event, which the solve between two pushes has passed)
+    int rc = poll_status(s, true);
+    if (rc) return rc;
     // the push uses whatever E the world holds (computeEF or a host write), as the
     // reference's advance() reads world.ef as it stands
     // room for one more step: output bins need the current population plus slack
@@ -1993,6 +2064,13 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     PartBuf &in = s->buf[s->cur], &out = s->buf[1 - s->cur];
     rc = ensure_push_scratch(s);
     if (rc) return rc;
+    // far deposits that no density finish consumed (a push or deposit pass not followed by
+    // computeNumberDensity) must not add to this push's density
+    if (s->extra_dirty) {
+        k_fill_f64<<<nb256(g.nn), 256, 0, w->stream>>>(g.nn, s->den_extra, 0.0);
+        PIC_LAUNCH_CHECK();
+        s->extra_dirty = false;
+    }
     // the push phase: segment plan of the output buffer, then the fused push
     hipEvent_t ev;
     timer_begin(w, PIC_T_PUSH, &ev);
@@ -2043,6 +2121,17 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     PIC_LAUNCH_CHECK();
     timer_end(w, PIC_T_PUSH_KERNEL, evk);
     timer_end(w, PIC_T_PUSH, ev);
+    // snapshot of this push, read by the host at its next call without a stream sync
+    k_push_status<<<1, 256, 0, w->stream>>>(s->diag_part + 5 * (size_t)grid, (int)grid, out.stage_count,
+                                           out.cap_stage, s->st_dev);
+    PIC_LAUNCH_CHECK();
+    PIC_HIP(hipMemcpyAsync(s->st_host, s->st_dev, sizeof(SpStatus), hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipEventRecord(s->st_ev, w->stream));
+    s->st_pending = true;
+    s->added_since_push = 0;
+    s->stage_known = false;
+    s->np_exact = true;
+    s->extra_dirty = true;
     s->cur = 1 - s->cur;
     s->den_state = 1;
     s->diag_valid = true;
@@ -2055,10 +2144,15 @@ int pic_species_compute_number_density(pic_species *s) {
     PIC_CHECK(s, PIC_ERR_ARG, "null argument");
     pic_world *w = s->w;
     const Grid &g = w->g;
+    // particles the last push dropped are reported here when its snapshot has landed
+    int rc = poll_status(s, false);
+    if (rc) return rc;
     if (s->den_state == 2) return PIC_OK;
     hipEvent_t ev;
     timer_begin(w, PIC_T_DEPOSIT, &ev);
     if (s->den_state == 0) {
+        if (s->extra_dirty)  // far deposits of a push whose density was never finished
+            k_fill_f64<<<nb256(g.nn), 256, 0, w->stream>>>(g.nn, s->den_extra, 0.0);
         k_fill_u8<<<nb256(g.ntiles), 256, 0, w->stream>>>(g.ntiles, s->win_used, 0);
         unsigned grid = g.ntiles + STAGE_WGS;
         PartView v = view_of(s->buf[s->cur]);
@@ -2079,6 +2173,7 @@ int pic_species_compute_number_density(pic_species *s) {
     PIC_LAUNCH_CHECK();
     timer_end(w, PIC_T_DEPOSIT, ev);
     s->den_state = 2;
+    s->extra_dirty = false;  // the finish consumed and zeroed den_extra
     return PIC_OK;
 }
 
@@ -2135,15 +2230,21 @@ static int inject_common(pic_species *s, InjectParams &I, bool flat_ok) {
     (void)flat_ok;
     pic_world *w = s->w;
     if (I.n <= 0) return PIC_OK;
-    std::vector<uint32_t> cnt;
     uint32_t stage = 0;
-    // the host knows an upper bound of the population; read the exact staging count
-    PIC_HIP(hipMemcpyAsync(&w->scal_host->iters, s->buf[s->cur].stage_count, 4, hipMemcpyDeviceToHost, w->stream));
-    PIC_HIP(hipStreamSynchronize(w->stream));
-    stage = (uint32_t)w->scal_host->iters;
+    // the staging count comes from the last push's snapshot when it has landed (the solve
+    // between two pushes has passed it), else from a synchronous read
+    int rc = poll_status(s, false);
+    if (rc) return rc;
+    if (s->stage_known) {
+        stage = (uint32_t)s->stage_host;
+    } else {
+        PIC_HIP(hipMemcpyAsync(&w->scal_host->iters, s->buf[s->cur].stage_count, 4, hipMemcpyDeviceToHost,
+                               w->stream));
+        PIC_HIP(hipStreamSynchronize(w->stream));
+        stage = (uint32_t)w->scal_host->iters;
+    }
     long n = I.n;
     long need_total = s->np_upper + n;
-    int rc = PIC_OK;
     if (stage + n > s->buf[s->cur].cap_stage || need_total + need_total / 4 > s->buf[s->cur].cap_bins) {
         rc = ensure_capacity(s, need_total, stage + n);
         if (rc) return rc;
@@ -2170,6 +2271,9 @@ static int inject_common(pic_species *s, InjectParams &I, bool flat_ok) {
     timer_end(w, PIC_T_INJECT, ev);
     s->next_id += (uint32_t)n;
     s->np_upper += n;
+    s->added_since_push += n;
+    s->stage_host = stage + n;
+    s->stage_known = true;
     s->stage_added += n;
     s->sorted = false;
     s->den_state = 0;

@@ -150,6 +150,15 @@ struct DevScalars {
     int pad;
 };
 
+// per-species device status word (see pic_species::st_dev)
+struct SpStatus {
+    uint32_t err;       // capacity flags raised since the last snapshot: 1 bins short, 2 particle dropped
+    uint32_t err_snap;  // flags taken (and cleared) by the last snapshot
+    uint32_t stage;     // staging count after the last push
+    uint32_t pad;
+    long long np;       // live particles after the last push
+};
+
 struct TimerRec {
     int phase;
     hipEvent_t a, b;
@@ -218,8 +227,18 @@ struct pic_species {
     int cur = 0;                  // buffer holding the current particles
     long np_upper = 0;            // host-side upper bound of the particle count
     uint32_t next_id = 0;
-    long *np_dev = nullptr;       // device particle counter written by the push
-    long *np_host = nullptr;      // pinned mirror
+    // Status of the last push, written in stream order after it (k_push_status) and copied
+    // to pinned memory behind st_ev: the live count, the staging count and the capacity
+    // flags that push raised.  The host reads it one call later without a stream sync.
+    pic::SpStatus *st_dev = nullptr;
+    pic::SpStatus *st_host = nullptr;  // pinned mirror
+    hipEvent_t st_ev = nullptr;
+    bool st_pending = false;      // a snapshot is in flight and not yet read
+    long added_since_push = 0;    // particles appended since the snapshotted push
+    bool stage_known = true;      // stage_host is the exact staging count
+    long stage_host = 0;
+    bool np_exact = true;         // np_upper is the exact count (no host edit since a count)
+    bool extra_dirty = false;     // den_extra holds deposits that no density finish consumed
     double *den = nullptr, *den_extra = nullptr, *den_ave = nullptr;
     int ave_samples = 0;
     void *win = nullptr;          // per-tile deposit windows (float or double)
@@ -258,6 +277,8 @@ struct pic_solver {
     double *boltz = nullptr;
     float *g32 = nullptr;  // FP32 copy of the CG residual, the V-cycle's right-hand side
     pic::MgHierarchy *mg = nullptr;
+    // what the last solve() did (pic_solver_get_info)
+    int info_newton = 0, info_iters = 0, info_fallbacks = 0, info_converged = 0;
 };
 
 // ---- error handling ------------------------------------------------------------------

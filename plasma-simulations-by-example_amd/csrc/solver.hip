@@ -1018,6 +1018,11 @@ static int ensure_pcg_vectors(pic_solver *s) {
     long nn = s->w->g.nn;
     double **v[7] = {&s->y, &s->g, &s->d, &s->z, &s->F, &s->P, &s->z2};
     for (auto p : v) PIC_HIP(hipMalloc(p, sizeof(double) * nn));
+    // y = 0 everywhere once: the multigrid path writes y on regular nodes only, and the
+    // Newton update reads the face nodes' chain targets, which may be Dirichlet nodes (as
+    // the reference zeroes y there, ch3/ver2/PotentialSolver.cpp:273-275)
+    k_fill<<<nbk(nn), 256, 0, s->w->stream>>>(nn, s->y, 0.0);
+    PIC_LAUNCH_CHECK();
     return PIC_OK;
 }
 
@@ -1048,9 +1053,11 @@ static int solve_nrpcg(pic_solver *s, double *residual, int *iters) {
         bool pc = false;
         rc = pcg_linear(s, Q, &pc, &pcg_its);
         if (rc) return rc;
+        s->info_newton = it + 1;
         if (!pc) {
             rc = gs_linear(s, Q);
             if (rc) return rc;
+            s->info_fallbacks++;
         }
         k_nr_update<<<nb8, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, w->partials);
         k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb8, &w->scal->red[5]);
@@ -1267,9 +1274,11 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
             rc = pcgs_linear(s, Q, &pc, &pcg_its);
         }
         if (rc) return rc;
+        s->info_newton = it + 1;
         if (!pc) {
             rc = gsls_linear(s, Q);
             if (rc) return rc;
+            s->info_fallbacks++;
         }
         k_nrs_update<<<nb8, 256, 0, w->stream>>>(Q, w->object_id, w->phi, s->y, d_pend, w->scal, w->partials);
         k_sum_to<<<1, 1024, 0, w->stream>>>(w->partials, (int)nb8, &w->scal->red[5]);
@@ -1346,6 +1355,10 @@ int pic_solver_solve(pic_solver *s, double *residual, int *iters) {
     hipEvent_t ev;
     timer_begin(w, PIC_T_SOLVE, &ev);
     int rc;
+    int its_local = 0;
+    if (!iters) iters = &its_local;
+    s->info_newton = 0;
+    s->info_fallbacks = 0;
     switch (s->type) {
     case PIC_SOLVER_SOR_LINEAR: rc = solve_sor<false>(s, residual, iters); break;
     case PIC_SOLVER_GS: rc = solve_sor<true>(s, residual, iters); break;
@@ -1359,7 +1372,18 @@ int pic_solver_solve(pic_solver *s, double *residual, int *iters) {
         break;
     }
     timer_end(w, PIC_T_SOLVE, ev);
+    s->info_iters = *iters;
+    s->info_converged = rc == PIC_OK ? 1 : 0;
     return rc;
+}
+
+int pic_solver_get_info(pic_solver *s, int info[4]) {
+    PIC_CHECK(s && info, PIC_ERR_ARG, "null argument");
+    info[0] = s->info_newton;
+    info[1] = s->info_iters;
+    info[2] = s->info_fallbacks;
+    info[3] = s->info_converged;
+    return PIC_OK;
 }
 
 int pic_solver_solve_qn(pic_solver *s) {

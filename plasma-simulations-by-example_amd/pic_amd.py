@@ -38,7 +38,7 @@ EXPORTS = [
     "pic_species_get_den_ave", "pic_species_update_averages", "pic_species_diag",
     "pic_species_sort", "pic_species_inject_beam", "pic_species_load_box_device",
     "pic_solver_create", "pic_solver_destroy", "pic_solver_set_reference_values",
-    "pic_solver_solve", "pic_solver_solve_qn", "pic_solver_compute_ef",
+    "pic_solver_solve", "pic_solver_get_info", "pic_solver_solve_qn", "pic_solver_compute_ef",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -117,6 +117,7 @@ def lib():
             "pic_solver_set_reference_values": (C.c_int, [_vp, C.c_double, C.c_double,
                                                           C.c_double]),
             "pic_solver_solve": (C.c_int, [_vp, _dp, C.POINTER(C.c_int)]),
+            "pic_solver_get_info": (C.c_int, [_vp, C.POINTER(C.c_int)]),
             "pic_solver_solve_qn": (C.c_int, [_vp]),
             "pic_solver_compute_ef": (C.c_int, [_vp]),
         }
@@ -357,6 +358,12 @@ class PotentialSolver:
         it = C.c_int(0)
         rc = _check(lib().pic_solver_solve(self.h, C.byref(r), C.byref(it)), allow_nc=True)
         return rc == PIC_OK, r.value, it.value
+
+    def info(self):
+        """What the last solve did: {"newton", "iters", "fallbacks", "converged"}."""
+        a = (C.c_int * 4)()
+        _check(lib().pic_solver_get_info(self.h, a))
+        return {"newton": a[0], "iters": a[1], "fallbacks": a[2], "converged": bool(a[3])}
 
     def solve_qn(self):
         _check(lib().pic_solver_solve_qn(self.h))

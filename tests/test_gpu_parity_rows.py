@@ -1,0 +1,271 @@
+"""GPU parity of the rows whose earlier tests left a path unchecked: the addParticle rewind
+with a live field, the PCG -> solveGSLinear fallback, a neutral among three species, the
+density after call orders other than advance -> computeNumberDensity, and the capacity
+errors and bounds of a run that never calls getNp.
+
+Every comparison is against the oracle (oracle/pic_oracle.c, pinned bit-for-bit to the
+compiled reference) on the same inputs.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pic_amd as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SX0, SXM = (-0.1, -0.1, 0.0), (0.1, 0.1, 0.4)
+AMU = O.AMU
+
+
+def sphere_world(n=(21, 21, 41)):
+    w = P.World(*n)
+    w.set_extents(SX0, SXM)
+    return w, O.OGrid(*n, SX0, SXM)
+
+
+def live_field(w, g, seed=11):
+    """A solved, non-zero potential around a -100 V sphere and its E (computeEF on the
+    GPU): the field the rewind must use."""
+    w.add_sphere((0, 0, 0.15), 0.05, -100.0)
+    w.add_inlet()
+    rng = np.random.default_rng(seed)
+    w.set_field(P.RHO, rng.uniform(0.2, 1.0, g.nn) * 1e10 * O.QE)
+    s = P.PotentialSolver(w, P.QN, 10, 1e-4)
+    s.set_reference_values(0.0, 1.5, 1e10)
+    s.solve()
+    s.compute_ef()
+    ef = w.field(P.EF)
+    assert np.abs(ef).max() > 100.0  # a live field, kV/m near the sphere
+    return s, ef
+
+
+def by_id(parts):
+    pos, vel, mpw, ids = parts
+    o = np.argsort(ids)
+    return pos[o], vel[o], mpw[o], ids[o]
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.MIXED])
+def test_add_particles_rewinds_with_live_field(prec):
+    """Species::addParticle (ch3/ver2/Species.cpp:65-81) after computeEF: v -= q/m E(x) dt/2
+    with the current field.  FP64: bit-identical to the oracle; mixed: FP32 gather of the
+    same field."""
+    w, g = sphere_world()
+    _, ef = live_field(w, g)
+    rng = np.random.default_rng(5)
+    n = 20000
+    pos = g.x0 + rng.random((n, 3)) * (g.xm - g.x0)
+    pos[:50, 0] = 0.1 + rng.random(50) * 1e-3  # outside: rejected by inBounds
+    vel = rng.normal(0, 3000, (n, 3))
+    mpw = rng.uniform(1, 2, n)
+    dt, m = 1e-7, 16 * AMU
+    sp = P.Species(w, m, O.QE, 1.0, prec, capacity=n, track_ids=True)
+    added = sp.add_particles(pos, vel, mpw, dt)
+    Po = O.Particles(n)
+    O.add_particles(g, Po, ef, dt, O.QE, m, pos, vel, mpw)
+    po, vo, wo = Po.view()
+    assert added == Po.n == n - 50
+    gp, gv, gw, ids = by_id(sp.particles())
+    assert np.array_equal(ids, np.arange(added))
+    rewind = vel[50:] - vo
+    assert np.abs(rewind).max() > 1.0  # the field moved the velocities
+    if prec == P.FP64:
+        assert np.array_equal(gp, po) and np.array_equal(gv, vo) and np.array_equal(gw, wo)
+    else:
+        np.testing.assert_allclose(gp, po, rtol=0, atol=1e-9)
+        # FP32 field at an FP32 offset, FP32 velocity: the rewind to 2e-6 of its size
+        err = np.abs(gv - vo).max()
+        assert err <= 2e-6 * np.abs(rewind).max() + 1e-6 * np.abs(vel).max(), err
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.MIXED])
+def test_inject_beam_rewinds_with_live_field(prec):
+    """ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27) on the device with a live field:
+    each injected particle's velocity is (0, 0, v) rewound through addParticle's rule at its
+    own position, as the oracle computes it from the same positions and field."""
+    w, g = sphere_world()
+    _, ef = live_field(w, g)
+    dt, v, m = 1e-7, 7000.0, 16 * AMU
+    sp = P.Species(w, m, O.QE, 20.0, prec, capacity=50000, track_ids=True)
+    n = sp.inject_beam(v, 1e10, 0.0, dt, seed=3, counter=0)
+    assert n > 10000
+    gp, gv, gw, _ = by_id(sp.particles())
+    Po = O.Particles(n)
+    O.add_particles(g, Po, ef, dt, O.QE, m, gp, np.tile([0.0, 0.0, v], (n, 1)), gw)
+    po, vo, _ = Po.view()
+    assert Po.n == n
+    assert np.abs(vo - [0, 0, v]).max() > 1.0  # the inlet plane sees the field
+    if prec == P.FP64:
+        assert np.array_equal(gv, vo)
+    else:
+        err = np.abs(gv - vo).max()
+        assert err <= 2e-6 * np.abs(vo - [0, 0, v]).max() + 1e-6 * v, err
+
+
+def test_gs_fallback_taken_and_solution_matches_oracle():
+    """solveNRPCG falls back to solveGSLinear when PCG does not converge within max_it
+    (ch3/ver2/PotentialSolver.cpp:270-271).  With PCG capped at 200 iterations this case
+    needs the fallback on both sides; the GPU reports that it took it, and both reach the
+    same Newton fixed point (the GPU's fallback sweeps red-black, the reference
+    lexicographically, so the iterates differ and only the converged potentials compare)."""
+    n = (13, 13, 25)
+    w, g = sphere_world(n)
+    w.add_sphere((0, 0, 0.15), 0.05, -1.0)
+    w.add_inlet()
+    rho = np.full(g.nn, 1e10 * O.QE)
+    w.set_field(P.RHO, rho)
+    s = P.PotentialSolver(w, P.PCG, 200, 1e-4)
+    s.set_reference_values(0.0, 1.5, 1e10)
+    s.solve_qn()
+    ok, _, _ = s.solve()
+    info = s.info()
+    oid = np.zeros(g.nn, dtype=np.int32)
+    phi = g.zeros()
+    O.add_sphere(g, oid, phi, (0, 0, 0.15), 0.05, -1.0)
+    O.add_inlet(g, oid, phi)
+    O.solve_qn(g, phi, rho, oid, 0, 1.5, 1e10)
+    ok_r, st = O.solve_nrpcg(g, phi, rho, oid, 0, 1.5, 1e10, 200, 1e-4)
+    assert ok_r and st[2] >= 1  # the oracle converged through at least one fallback
+    assert info["fallbacks"] >= 1, info  # the GPU took the fallback path too
+    assert ok and info["converged"], info
+    # both stop when the RMS Newton correction is below NR_TOL = 1e-3 V
+    np.testing.assert_allclose(w.field(P.PHI), phi, rtol=0, atol=5e-3)
+    # a solve that converges without the fallback reports none
+    s2 = P.PotentialSolver(w, P.PCG, 20000, 1e-4)
+    s2.set_reference_values(0.0, 1.5, 1e10)
+    s2.solve()
+    assert s2.info()["fallbacks"] == 0
+
+
+def test_three_species_with_neutral_charge_density():
+    """ch9/MT's O, O+, O++ (ch9/MT/Main.cpp:38-40): computeChargeDensity skips the neutral
+    (World.cpp:52); the charged ones add q*den.  The neutral pushes ballistically."""
+    w, g = sphere_world((17, 17, 33))
+    rng = np.random.default_rng(8)
+    nv = g.node_volumes()
+    phi = rng.normal(0, 5, g.nn)
+    w.set_field(P.PHI, phi)
+    s = P.PotentialSolver(w, P.SOR_LINEAR, 10, 1e-2)
+    s.compute_ef()
+    ef = w.field(P.EF)
+    m = 16 * AMU
+    charges = [0.0, O.QE, 2 * O.QE]
+    sps, orc = [], []
+    for q in charges:
+        pos = g.x0 + rng.random((30000, 3)) * (g.xm - g.x0)
+        vel = rng.normal(0, 2000, (30000, 3))
+        mpw = rng.uniform(1, 3, 30000) * 1e3
+        sp = P.Species(w, m, q, 1e3, P.FP64, capacity=40000)
+        sp.set_particles(pos, vel, mpw)
+        Po = O.Particles(40000)
+        Po.set(pos, vel, mpw)
+        sps.append(sp)
+        orc.append(Po)
+    dt = 1e-7
+    for _ in range(3):
+        dens = []
+        for sp, Po, q in zip(sps, orc, charges):
+            sp.advance(P.ABSORB, dt)
+            sp.compute_number_density()
+            O.push_absorb(g, Po, ef, dt, q, m, (10.0, 10.0, 10.0), 0.0)  # no sphere
+            dens.append(O.number_density(g, Po.pos[:Po.n], Po.mpw[:Po.n], nv))
+        w.compute_charge_density(sps)
+        rho = O.charge_density(charges, dens)
+        np.testing.assert_allclose(w.field(P.RHO), rho, rtol=1e-12, atol=1e-13 * np.abs(rho).max())
+        for sp, d in zip(sps, dens):
+            np.testing.assert_allclose(sp.den(), d, rtol=1e-12, atol=1e-13 * np.abs(d).max())
+    # the neutral moved in straight lines: its velocities are the ones it started with
+    _, vel0, _, _ = sps[0].particles()
+    assert np.array_equal(np.sort(vel0[:, 0]), np.sort(orc[0].vel[:orc[0].n, 0]))
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.MIXED])
+@pytest.mark.parametrize("order", ["advance-advance", "advance-inject", "advance-add"])
+def test_density_after_unfinished_push(prec, order):
+    """computeNumberDensity clears den and rescatters the current particles
+    (ch3/ver2/Species.cpp:51-62) whatever came before it: far deposits of a push whose
+    density was never finished must not leak into the next density."""
+    w, g = sphere_world((17, 17, 33))
+    nv = g.node_volumes()
+    rng = np.random.default_rng(13)
+    n = 60000
+    pos = g.x0 + rng.random((n, 3)) * (g.xm - g.x0)
+    dh = g.dh[2]
+    dt = 1e-7
+    # fast particles: many land beyond the deposit window (far deposits)
+    vel = rng.normal(0, 4 * dh / dt, (n, 3))
+    sp = P.Species(w, 16 * AMU, O.QE, 1.0, prec, capacity=2 * n)
+    w.set_field(P.EF, np.zeros(3 * g.nn))
+    sp.set_particles(pos, vel, np.ones(n))
+    sp.sort()
+    sp.advance(P.ABSORB, dt)
+    if order == "advance-advance":
+        sp.advance(P.ABSORB, dt)
+    elif order == "advance-inject":
+        assert sp.inject_beam(7000.0, 1e10, 0.0, dt, seed=1, counter=0) > 200
+    else:
+        extra = g.x0 + rng.random((5000, 3)) * (g.xm - g.x0)
+        sp.add_particles(extra, rng.normal(0, 1000, (5000, 3)), np.ones(5000), dt)
+        sp.advance(P.ABSORB, dt)
+    sp.compute_number_density()
+    gp, _, gw, _ = sp.particles()
+    ref = O.number_density(g, gp, gw, nv)
+    tol = 1e-12 if prec == P.FP64 else 2e-6
+    np.testing.assert_allclose(sp.den(), ref, rtol=tol, atol=tol * np.abs(ref).max())
+    assert abs(np.sum(sp.den() * nv) / gw.sum() - 1) < (1e-12 if prec == P.FP64 else 1e-6)
+
+
+def test_staging_overflow_reported_at_next_call_without_getnp():
+    """A push whose staging list fills up drops particles; the next advance reports
+    PIC_ERR_CAPACITY (one call later, with no getNp in between)."""
+    n_nodes = (41, 9, 9)
+    w = P.World(*n_nodes)
+    w.set_extents((0.0, 0.0, 0.0), (0.4, 0.08, 0.08))
+    w.set_field(P.EF, np.zeros(3 * np.prod(n_nodes)))
+    dh = 0.01
+    dt = 1e-7
+    n = 200000
+    rng = np.random.default_rng(2)
+    # every particle in tile 0, jumping 20 cells in +x in one step: into an empty tile two
+    # tiles away whose inflow room is tiny, and past the 65536-slot staging list
+    pos = np.column_stack([rng.uniform(0.001, 0.079, n), rng.uniform(0.001, 0.079, n),
+                           rng.uniform(0.001, 0.079, n)])
+    vel = np.tile([20 * dh / dt, 0.0, 0.0], (n, 1))
+    sp = P.Species(w, 16 * AMU, O.QE, 1.0, P.FP64, capacity=n)
+    sp.set_particles(pos, vel, np.ones(n))
+    sp.sort()
+    sp.advance(P.REFLECT, dt)  # drops particles; the error is not known yet
+    with pytest.raises(P.PicError, match="capacity"):
+        sp.advance(P.REFLECT, dt)
+
+
+def _dev_free_bytes():
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_capacity_bounded_when_injecting_without_getnp():
+    """Injection every step with absorbing walls and no getNp: the live population levels
+    off, and so must the particle buffers (the push's own count replaces the cumulative
+    injected count as the host's bound)."""
+    w, g = sphere_world((17, 17, 33))
+    w.set_field(P.EF, np.zeros(3 * g.nn))
+    sp = P.Species(w, 16 * AMU, O.QE, 20.0, P.MIXED, capacity=1000)
+    dt, v = 1e-6, 7000.0  # 0.007 m per step: a particle crosses the 0.4 m box in ~57 steps
+    free = []
+    for step in range(400):
+        sp.inject_beam(v, 1e10, 0.0, dt, seed=4, counter=step)
+        sp.advance(P.ABSORB, dt)
+        sp.compute_number_density()
+        if step in (150, 399):
+            w.sync()
+            free.append(_dev_free_bytes())
+    n_live = sp.np()
+    injected_per_step = 1e10 * v * 0.04 * dt / 20.0
+    assert n_live < 70 * injected_per_step  # levelled off at ~57 steps' worth
+    assert free[0] - free[1] < 16 << 20, (free, n_live)  # no growth after step 150
