@@ -1350,6 +1350,8 @@ template <int PREC>
 __global__ void __launch_bounds__(256)
     k_inject(InjectParams I, const float4 *ef32, const double4 *ef64, PartView dst, uint32_t stage0,
              long flat0, uint32_t *err) {
+    // the rewind keeps addParticle's rounding (ch3/ver2/Species.cpp:75-79): no FMA
+#pragma clang fp contract(off)
     long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (q >= I.n) return;
     const Grid &g = I.g;

@@ -834,8 +834,14 @@ __global__ void __launch_bounds__(256)
         node_ijk(g, u, i, j, k);
         int t = node_type(g, oid, u, i, j, k);
         double yy = 0;
-        if (t == NT_REG) yy = yv(u);
-        else if (t == NT_NEU) yy = yv(chain_target(g, oid, u, i, j, k));
+        if (t == NT_REG) {
+            yy = yv(u);
+        } else if (t == NT_NEU) {
+            // a chain that ends on the sphere or the inlet (a Dirichlet node, which the solve
+            // never updates and whose y is never written) leaves the face node as it is
+            long v = chain_target(g, oid, u, i, j, k);
+            if (oid[v] == 0) yy = yv(v);
+        }
         x[u] = x[u] - yy;
         y2 = yy * yy;
     }
