@@ -29,8 +29,8 @@
 extern "C" {
 #endif
 
-/* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL); 3: pic_solver_get_info */
-#define PICAMD_ABI_VERSION 3
+/* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL); 3: pic_solver_get_info; 4: pic_adjoint_* */
+#define PICAMD_ABI_VERSION 4
 
 /* status codes */
 #define PIC_OK 0
@@ -201,6 +201,55 @@ int pic_solver_get_info(pic_solver *s, int info[4]);
 int pic_solver_solve_qn(pic_solver *s);
 /* computeEF() (ch3/ver2/PotentialSolver.cpp:465-504). */
 int pic_solver_compute_ef(pic_solver *s);
+
+/* ---- Differentiable flow past a sphere and its adjoint ----------------------------------
+ * No counterpart in the reference: K steps of the ch3/ver2 loop (ch3/ver2/Main.cpp:50-70:
+ * inject with the addParticle rewind, advance, computeNumberDensity, computeChargeDensity,
+ * the Newton solve of the Boltzmann-electron Poisson equation, computeEF) in FP64, as a
+ * differentiable function of theta = (drift speed [m/s], sphere potential [V], Te [eV]), with
+ * a hand-written adjoint (reverse leapfrog, transposes of the CIC gather and scatter and of
+ * computeEF, one linear solve per step with the symmetric Newton Jacobian of the reduced
+ * system) over checkpoints stored every checkpoint_every steps.  Differences from the
+ * reference loop that make it differentiable: absorption at the sphere and the five open
+ * walls multiplies a particle's weight by clamp(distance/(soft_width*dh), 0, 1) (0 inside,
+ * as inSphere || !inBounds kills it) instead of killing it outright; each step injects
+ * n_inject particles whose weight n0*v_drift*A*dt/n_inject carries the flux; random numbers
+ * are Philox keyed by (seed, slot, step).  The world is config's sphere domain with an inlet
+ * at k = 0; the solve is Newton + multigrid-preconditioned CG (PIC_SOLVER_MG). */
+typedef struct pic_adjoint pic_adjoint;
+typedef struct {
+    int ni, nj, nk;
+    double x0[3], xm[3];
+    double sphere_c[3], sphere_r;
+    double dt;
+    int steps;             /* K */
+    int checkpoint_every;  /* M */
+    double mass, charge;   /* the ion species */
+    double n0;             /* ion density of the initial fill and the beam; Boltzmann n0 */
+    double T_ion;          /* ion temperature [K] of the fill and the beam (0: cold) */
+    long n_init;           /* macro-particles of the initial uniform fill */
+    long n_inject;         /* macro-particles injected per step */
+    double soft_width;     /* width of the soft absorbing layers, in cells */
+    double cg_tol;         /* CG tolerance of each Newton step (<= 0: 1e-6) */
+    double nr_tol;         /* Newton stopping test, RMS of the correction [V] (<= 0: 1e-8) */
+    double adj_rel_tol;    /* adjoint linear solves, relative to the RHS RMS (<= 0: 1e-8) */
+    uint64_t seed;
+} pic_adjoint_config;
+int pic_adjoint_create(const pic_adjoint_config *cfg, pic_adjoint **out);
+int pic_adjoint_destroy(pic_adjoint *a);
+/* Runs the K steps at theta.  current (nullable): the ion current to the sphere,
+ * charge * sum over steps of the weight the sphere took / (K dt) [A].  den_out, phi_out
+ * (nullable): DEVICE pointers, nn doubles each: the ion number density and the potential
+ * after step K. */
+int pic_adjoint_forward(pic_adjoint *a, const double theta[3], double *current, double *den_out,
+                        double *phi_out);
+/* Gradient of L at the last forward's theta, for dL/dcurrent = dcurrent, dL/dden = dden and
+ * dL/dphi = dphi (DEVICE pointers, nn doubles each; null for 0).  grad[3] = dL/dtheta. */
+int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, const double *dphi,
+                         double grad[3]);
+/* out[4]: last forward ms, last backward ms, Newton solves of the last forward, CG iterations
+ * of the last backward's linear solves. */
+int pic_adjoint_stats(pic_adjoint *a, double out[4]);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,28 @@ class MT19937(C.Structure):
     _fields_ = [("mt", C.c_uint32 * 624), ("idx", C.c_int)]
 
 
+class DsphConfig(C.Structure):
+    """orc_dsph_config (field order of the GPU's pic_adjoint_config)."""
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+                ("x0", C.c_double * 3), ("xm", C.c_double * 3),
+                ("sphere_c", C.c_double * 3), ("sphere_r", C.c_double),
+                ("dt", C.c_double), ("steps", C.c_int), ("checkpoint_every", C.c_int),
+                ("mass", C.c_double), ("charge", C.c_double), ("n0", C.c_double),
+                ("T_ion", C.c_double), ("n_init", C.c_long), ("n_inject", C.c_long),
+                ("soft_width", C.c_double), ("cg_tol", C.c_double), ("nr_tol", C.c_double),
+                ("adj_rel_tol", C.c_double), ("seed", C.c_uint64)]
+
+
+def dsph_config(**kw):
+    c = DsphConfig()
+    for k, v in kw.items():
+        if k in ("x0", "xm", "sphere_c"):
+            getattr(c, k)[:] = [float(t) for t in v]
+        else:
+            setattr(c, k, v)
+    return c
+
+
 _lib = None
 
 
@@ -110,6 +132,12 @@ def lib():
             "orc_mt_seed": (None, [C.POINTER(MT19937), C.c_uint32]),
             "orc_mt_next": (C.c_uint32, [C.POINTER(MT19937)]),
             "orc_mt_uniform": (C.c_double, [C.POINTER(MT19937)]),
+            "orc_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
+                                         C.POINTER(C.c_uint32)]),
+            "orc_solve_nr_sym": (C.c_int, [C.POINTER(Grid), _dp, _dp, _ip, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_int, _dp]),
+            "orc_dsph_run": (C.c_int, [C.POINTER(DsphConfig), _dp, _dp, _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -340,3 +368,35 @@ def species_diag(vel, mpw, mass):
     mpw = np.ascontiguousarray(mpw, dtype=np.float64)
     lib().orc_species_diag(len(mpw), _d(vel), _d(mpw), mass, _d(out))
     return out
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 of a 4-word counter under a 2-word key."""
+    c = (C.c_uint32 * 4)(*[int(t) & 0xFFFFFFFF for t in ctr])
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF, out)
+    return list(out)
+
+
+def solve_nr_sym(g: OGrid, phi, rho, object_id, phi0, Te0, n0, cg_tol, nr_tol, max_cg=20000):
+    """Newton on the Neumann-reduced system with Jacobi-CG; phi updated in place.
+    Returns (converged, [newton its, cg its, 0, last |y|rms])."""
+    st = np.zeros(4)
+    oid = np.ascontiguousarray(object_id, dtype=np.int32)
+    ok = lib().orc_solve_nr_sym(g.ref, _d(phi), _d(np.ascontiguousarray(rho)), _i(oid),
+                                phi0, Te0, n0, cg_tol, nr_tol, max_cg, _d(st))
+    return bool(ok), st
+
+
+def dsph_run(cfg: DsphConfig, theta):
+    """The differentiable sphere run at theta = (drift, sphere potential, Te).  Returns
+    (current, den and phi after the last step, [solves, cg its, live weight, sphere weight])."""
+    nn = cfg.ni * cfg.nj * cfg.nk
+    den, phi = np.zeros(nn), np.zeros(nn)
+    cur = C.c_double(0)
+    st = np.zeros(4)
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    ok = lib().orc_dsph_run(C.byref(cfg), _d(th), C.byref(cur), _d(den), _d(phi), _d(st))
+    if not ok:
+        raise RuntimeError("oracle differentiable run: a Newton solve did not converge")
+    return cur.value, den, phi, st

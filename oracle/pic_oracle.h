@@ -133,6 +133,37 @@ void orc_mt_seed(orc_mt19937 *r, uint32_t seed);
 uint32_t orc_mt_next(orc_mt19937 *r);
 double orc_mt_uniform(orc_mt19937 *r);
 
+/* ---- differentiable flow past a sphere (pic_oracle_diff.c) --------------------------------
+ * The checker of the GPU's pic_adjoint_* (no reference counterpart): the same loop, soft
+ * absorption, injection and Philox draws, in FP64 on the CPU.  Field order as
+ * pic_adjoint_config. */
+typedef struct {
+    int ni, nj, nk;
+    double x0[3], xm[3];
+    double sphere_c[3], sphere_r;
+    double dt;
+    int steps, checkpoint_every;
+    double mass, charge, n0, T_ion;
+    long n_init, n_inject;
+    double soft_width, cg_tol, nr_tol, adj_rel_tol;
+    uint64_t seed;
+} orc_dsph_config;
+
+/* Philox4x32-10 (Salmon et al., SC'11) */
+void orc_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]);
+/* Newton on the Neumann-reduced system (the fixed point of solveNRPCG,
+ * ch3/ver2/PotentialSolver.cpp:225-296) with Jacobi-preconditioned CG from y = 0 until
+ * RMS(J y - F) < cg_tol, until RMS(y) < nr_tol.  stats: Newton its, CG its, 0, last norm. */
+int orc_solve_nr_sym(const orc_grid *g, double *phi, const double *rho, const int *object_id,
+                     double phi0, double Te0, double n0, double cg_tol, double nr_tol, int max_cg,
+                     double *stats);
+/* K steps at theta = (drift, sphere potential, Te).  current: the ion current to the sphere;
+ * den_out, phi_out (nullable, nn): density and potential after the last step; stats
+ * (nullable, 4): solves, CG its, live weight at the end, weight taken by the sphere.
+ * Returns 1 if every solve converged. */
+int orc_dsph_run(const orc_dsph_config *c, const double theta[3], double *current, double *den_out,
+                 double *phi_out, double *stats);
+
 #ifdef __cplusplus
 }
 #endif

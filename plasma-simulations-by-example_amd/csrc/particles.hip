@@ -1872,10 +1872,13 @@ static int append_host(pic_species *s, long n, const double *pos, const double *
     if (rc) return rc;
     rc = count_particles(s, cnt, stage);  // a regrow sorts staging into the bins
     if (rc) return rc;
+    inbins = 0;
+    for (auto c : cnt) inbins += c;
     if (s->buf[s->cur].cap_stage < stage + n) {
         // staging still short after growth: sort what is there into the bins first
         rc = do_sort(s);
         if (rc) return rc;
+        inbins += stage;
         stage = 0;
     }
     PartBuf &b = s->buf[s->cur];

@@ -279,6 +279,9 @@ struct pic_solver {
     pic::MgHierarchy *mg = nullptr;
     // what the last solve() did (pic_solver_get_info)
     int info_newton = 0, info_iters = 0, info_fallbacks = 0, info_converged = 0;
+    // Newton stopping test of the reduced solvers: RMS(y) < nr_tol (the reference's NR_TOL,
+    // ch3/ver2/PotentialSolver.cpp:229; tightened by the differentiable run)
+    double nr_tol = 1e-3;
 };
 
 // ---- error handling ------------------------------------------------------------------

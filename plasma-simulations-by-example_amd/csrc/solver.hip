@@ -1256,7 +1256,7 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     k_neumann_copy<<<nb8, 256, 0, w->stream>>>(w->g, w->object_id, w->phi);
     if (!use_mg) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
     const int NR_MAX_IT = 20;
-    const double NR_TOL = 1e-3;
+    const double NR_TOL = s->nr_tol;  // 1e-3 as the reference (:229) unless a caller tightened it
     bool converged = false;
     double norm = 0;
     int pcg_its = 0;
@@ -1306,6 +1306,70 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
     if (iters) *iters = pcg_its;
     w->ef_valid = false;
     return converged ? PIC_OK : PIC_NOT_CONVERGED;
+}
+
+// Right-hand side of a linearised solve: F = rhs on the regular nodes (0 elsewhere) and the
+// CG start from y = 0, g = -F (FP64 and the V-cycle's FP32 copy)
+__global__ void __launch_bounds__(256)
+    k_lin_rhs(long nn, const uint8_t *mask, const double *rhs, double *F, double *gv, float *g32) {
+    long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (u >= nn) return;
+    double f = (mask[u] & 64) ? rhs[u] : 0.0;
+    F[u] = f;
+    gv[u] = -f;
+    if (g32) g32[u] = (float)(-f);
+}
+
+// Solves J lam = rhs on the regular nodes of the reduced system, J the Newton Jacobian at the
+// world's current phi (A_r - diag(P), symmetric negative definite), by the solver's own CG
+// (multigrid-preconditioned for PIC_SOLVER_MG, Jacobi for PIC_SOLVER_PCG_SYM) until
+// RMS(J lam - rhs) < tol.  lam is 0 off the regular nodes.  This is the linear solve of the
+// adjoint of solve(): J is symmetric, so J^T lam = rhs is this system.
+int solve_linearized(pic_solver *s, const double *rhs, double *lam, double tol, int *iters) {
+    pic_world *w = s->w;
+    PIC_CHECK(s->type == PIC_SOLVER_MG || s->type == PIC_SOLVER_PCG_SYM, PIC_ERR_STATE,
+              "linearised solves need the reduced symmetric system (PIC_SOLVER_MG or PIC_SOLVER_PCG_SYM)");
+    int rc = ensure_pcg_vectors(s);
+    if (rc) return rc;
+    if (!s->mask) PIC_HIP(hipMalloc(&s->mask, w->g.nn));
+    const bool use_mg = s->type == PIC_SOLVER_MG;
+    if (use_mg) {
+        rc = ensure_mg(s);
+        if (rc) return rc;
+    }
+    PcgParams Q = pcg_params(s);
+    Q.tol = tol;
+    long nn = w->g.nn;
+    unsigned nb8 = nbk8(nn);
+    rc = grow_partials(w, (int)nb8);
+    if (rc) return rc;
+    CgStart cs{nullptr, nullptr, nullptr};
+    if (use_mg) cs = CgStart{s->g, s->g32, mg_level0_idiag(s->mg)};
+    k_nrs_F<<<nb8, 256, 0, w->stream>>>(Q, w->phi, w->rho, w->object_id, s->F, s->P, s->mask, s->boltz, cs);
+    k_lin_rhs<<<nbk(nn), 256, 0, w->stream>>>(nn, s->mask, rhs, s->F, s->g, use_mg ? s->g32 : nullptr);
+    PIC_LAUNCH_CHECK();
+    bool conv = false;
+    int its = 0;
+    if (use_mg) {
+        rc = mg_setup(s->mg, Q, s->boltz, w->geom_stamp, w->stream);
+        if (rc) return rc;
+        const double *d_pend = nullptr;
+        rc = mgpcg_linear(s, Q, &conv, &its, &d_pend);
+        if (rc) return rc;
+        if (d_pend) k_pcg_finish<<<nb8, 256, 0, w->stream>>>(nn, d_pend, s->y, w->scal);
+    } else {
+        k_fill<<<nbk(nn), 256, 0, w->stream>>>(nn, s->y, 0.0);
+        rc = pcgs_linear(s, Q, &conv, &its);
+        if (rc) return rc;
+    }
+    PIC_LAUNCH_CHECK();
+    PIC_HIP(hipMemcpyAsync(lam, s->y, sizeof(double) * nn, hipMemcpyDeviceToDevice, w->stream));
+    if (iters) *iters = its;
+    if (!conv) {
+        set_error("linearised solve did not reach its tolerance within max_it CG iterations");
+        return PIC_NOT_CONVERGED;
+    }
+    return PIC_OK;
 }
 
 }  // namespace pic

@@ -39,7 +39,21 @@ EXPORTS = [
     "pic_species_sort", "pic_species_inject_beam", "pic_species_load_box_device",
     "pic_solver_create", "pic_solver_destroy", "pic_solver_set_reference_values",
     "pic_solver_solve", "pic_solver_get_info", "pic_solver_solve_qn", "pic_solver_compute_ef",
+    "pic_adjoint_create", "pic_adjoint_destroy", "pic_adjoint_forward", "pic_adjoint_backward",
+    "pic_adjoint_stats",
 ]
+
+
+class AdjointConfig(C.Structure):
+    """pic_adjoint_config (include/picamd.h)."""
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+                ("x0", C.c_double * 3), ("xm", C.c_double * 3),
+                ("sphere_c", C.c_double * 3), ("sphere_r", C.c_double),
+                ("dt", C.c_double), ("steps", C.c_int), ("checkpoint_every", C.c_int),
+                ("mass", C.c_double), ("charge", C.c_double), ("n0", C.c_double),
+                ("T_ion", C.c_double), ("n_init", C.c_long), ("n_inject", C.c_long),
+                ("soft_width", C.c_double), ("cg_tol", C.c_double), ("nr_tol", C.c_double),
+                ("adj_rel_tol", C.c_double), ("seed", C.c_uint64)]
 
 _dp = C.POINTER(C.c_double)
 _lp = C.POINTER(C.c_long)
@@ -120,6 +134,11 @@ def lib():
             "pic_solver_get_info": (C.c_int, [_vp, C.POINTER(C.c_int)]),
             "pic_solver_solve_qn": (C.c_int, [_vp]),
             "pic_solver_compute_ef": (C.c_int, [_vp]),
+            "pic_adjoint_create": (C.c_int, [C.POINTER(AdjointConfig), C.POINTER(_vp)]),
+            "pic_adjoint_destroy": (C.c_int, [_vp]),
+            "pic_adjoint_forward": (C.c_int, [_vp, _dp, _dp, _vp, _vp]),
+            "pic_adjoint_backward": (C.c_int, [_vp, C.c_double, _vp, _vp, _dp]),
+            "pic_adjoint_stats": (C.c_int, [_vp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -370,3 +389,58 @@ class PotentialSolver:
 
     def compute_ef(self):
         _check(lib().pic_solver_compute_ef(self.h))
+
+
+class SphereAdjointRun:
+    """The differentiable flow past a sphere (pic_adjoint_*): K steps of the ch3/ver2 loop in
+    FP64 with soft absorption, and their hand-written adjoint over checkpoints.
+
+    forward(theta, den_ptr, phi_ptr) runs the steps at theta = (drift [m/s], sphere
+    potential [V], Te [eV]) and returns the ion current to the sphere; den_ptr / phi_ptr
+    (device addresses of nn doubles, or 0) receive the density and potential after the last
+    step.  backward(dcurrent, dden_ptr, dphi_ptr) returns dL/dtheta at the last forward's
+    theta.  The torch.autograd.Function in adjoint.py wraps these two calls."""
+
+    def __init__(self, **cfg):
+        c = AdjointConfig()
+        for k, v in cfg.items():
+            if k in ("x0", "xm", "sphere_c"):
+                getattr(c, k)[:] = [float(t) for t in v]
+            else:
+                setattr(c, k, v)
+        self.cfg = c
+        self.nn = c.ni * c.nj * c.nk
+        self.h = _vp()
+        _check(lib().pic_adjoint_create(C.byref(c), C.byref(self.h)))
+        self.theta = None
+
+    def close(self):
+        if self.h and not _exiting:
+            lib().pic_adjoint_destroy(self.h)
+        self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, theta, den_ptr=0, phi_ptr=0):
+        th = np.ascontiguousarray(theta, dtype=np.float64).reshape(3)
+        cur = C.c_double(0)
+        _check(lib().pic_adjoint_forward(self.h, _d(th), C.byref(cur), _vp(den_ptr or None),
+                                         _vp(phi_ptr or None)))
+        self.theta = th.copy()
+        return cur.value
+
+    def backward(self, dcurrent, dden_ptr=0, dphi_ptr=0):
+        g = np.zeros(3)
+        _check(lib().pic_adjoint_backward(self.h, float(dcurrent), _vp(dden_ptr or None),
+                                          _vp(dphi_ptr or None), _d(g)))
+        return g
+
+    def stats(self):
+        """[forward ms, backward ms, Newton solves of the forward, adjoint CG iterations]"""
+        out = np.zeros(4)
+        _check(lib().pic_adjoint_stats(self.h, _d(out)))
+        return out
