@@ -163,6 +163,14 @@ int pic_species_update_averages(pic_species *s);
 /* Species::getRealCount / getMomentum / getKE (ch3/ver2/Species.cpp:84-108), FP64
  * reductions.  out[5] = real count, px, py, pz, KE. */
 int pic_species_diag(pic_species *s, double out[5]);
+/* No counterpart in the reference.  For callers that cannot allocate device or pinned memory
+ * (HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * outputs as pic_species_diag's full reduction over all particles, with the partial sums in a
+ * static device array of the library (6.3 MB per GPU, held until the process ends) instead
+ * of the species' own buffers.  It allocates nothing.  Grids of more than 131008 tiles
+ * (8x8x8 cells each) return PIC_ERR_CAPACITY.  Calls on one GPU share that array: calls
+ * that overlap on one GPU (several host threads or streams at once) race. */
+int pic_species_diag_noalloc(pic_species *s, double out[5]);
 /* Counting sort of all particles into tile bins (tile-major cell order). */
 int pic_species_sort(pic_species *s);
 /* ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27) when T == 0, WarmBeamSource::sample

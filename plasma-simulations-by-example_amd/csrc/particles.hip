@@ -12,6 +12,7 @@
 #include <type_traits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "picamd_internal.h"
 
@@ -1152,6 +1153,37 @@ __global__ void __launch_bounds__(1024) k_reduce_slots(const double *part, int n
     }
 }
 
+// Scratch of pic_species_diag_noalloc: NDIAG_SLOTS rows of per-block partials for up to
+// DIAG_NOALLOC_MAX_BLOCKS workgroups, then the NDIAG_SLOTS reduced sums (6.3 MB).  Each GPU
+// holds its own copy for the life of the process.  Host code never passes `scratch` itself
+// to a kernel: its device address comes from hipGetSymbolAddress, per device.
+constexpr long DIAG_NOALLOC_MAX_BLOCKS = 131072;
+constexpr long DIAG_NOALLOC_SUMS = NDIAG_SLOTS * DIAG_NOALLOC_MAX_BLOCKS;
+static __device__ __align__(256) double scratch[DIAG_NOALLOC_SUMS + 8];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// device address of `scratch` on device dev, looked up once per device with dev current
+static int scratch_on_device(int dev, double **out) {
+    PIC_CHECK(dev >= 0 && dev < SCRATCH_MAX_DEVICES, PIC_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        PIC_HIP(hipGetDevice(&cur));
+        if (cur != dev) PIC_HIP(hipSetDevice(dev));
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) PIC_HIP(hipSetDevice(cur));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        PIC_CHECK(p != nullptr, PIC_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PIC_OK;
+}
+
 // ---------------------------------------------------------------------------------------
 // counting sort: histogram -> plan (mode 1) -> scatter
 // ---------------------------------------------------------------------------------------
@@ -2228,6 +2260,44 @@ int pic_species_diag(pic_species *s, double out[5]) {
     out[2] = s->diag_host[2] * s->mass;
     out[3] = s->diag_host[3] * s->mass;
     out[4] = 0.5 * s->mass * s->diag_host[4];
+    return PIC_OK;
+}
+
+// pic_species_diag's full reduction (k_diag_all, then the fixed-order k_reduce_slots) with
+// its partials and sums in the static `scratch` of this device instead of the species'
+// buffers: no allocation, no pinned memory.  Calls overlapping on one GPU share `scratch`.
+int pic_species_diag_noalloc(pic_species *s, double out[5]) {
+    PIC_CHECK(s && out, PIC_ERR_ARG, "null argument");
+    pic_world *w = s->w;
+    const long nb = (long)w->g.ntiles + STAGE_WGS;
+    if (nb > DIAG_NOALLOC_MAX_BLOCKS) {
+        char msg[192];
+        snprintf(msg, sizeof msg,
+                 "pic_species_diag_noalloc: %ld tiles exceed its static scratch (at most %ld tiles); "
+                 "use pic_species_diag",
+                 (long)w->g.ntiles, DIAG_NOALLOC_MAX_BLOCKS - STAGE_WGS);
+        set_error(msg);
+        return PIC_ERR_CAPACITY;
+    }
+    double *ptr = nullptr;
+    int rc = scratch_on_device(w->device, &ptr);
+    if (rc) return rc;
+    PartView v = view_of(s->buf[s->cur]);
+    if (s->prec == PIC_MIXED)
+        k_diag_all<PIC_MIXED><<<(unsigned)nb, 256, 0, w->stream>>>(w->g, v, ptr);
+    else
+        k_diag_all<PIC_FP64><<<(unsigned)nb, 256, 0, w->stream>>>(w->g, v, ptr);
+    PIC_LAUNCH_CHECK();
+    k_reduce_slots<<<1, 1024, 0, w->stream>>>(ptr, (int)nb, NDIAG_SLOTS, ptr + DIAG_NOALLOC_SUMS);
+    PIC_LAUNCH_CHECK();
+    double h[NDIAG_SLOTS];
+    PIC_HIP(hipMemcpyAsync(h, ptr + DIAG_NOALLOC_SUMS, sizeof h, hipMemcpyDeviceToHost, w->stream));
+    PIC_HIP(hipStreamSynchronize(w->stream));
+    out[0] = h[0];
+    out[1] = h[1] * s->mass;
+    out[2] = h[2] * s->mass;
+    out[3] = h[3] * s->mass;
+    out[4] = 0.5 * s->mass * h[4];
     return PIC_OK;
 }
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "pic_species_create", "pic_species_destroy", "pic_species_add_particles",
     "pic_species_set_particles", "pic_species_get_particles", "pic_species_get_np",
     "pic_species_advance", "pic_species_compute_number_density", "pic_species_get_den",
-    "pic_species_get_den_ave", "pic_species_update_averages", "pic_species_diag",
+    "pic_species_get_den_ave", "pic_species_update_averages", "pic_species_diag", "pic_species_diag_noalloc",
     "pic_species_sort", "pic_species_inject_beam", "pic_species_load_box_device",
     "pic_solver_create", "pic_solver_destroy", "pic_solver_set_reference_values",
     "pic_solver_solve", "pic_solver_get_info", "pic_solver_solve_qn", "pic_solver_compute_ef",
@@ -121,6 +121,7 @@ def lib():
             "pic_species_get_den_ave": (C.c_int, [_vp, _dp]),
             "pic_species_update_averages": (C.c_int, [_vp]),
             "pic_species_diag": (C.c_int, [_vp, _dp]),
+            "pic_species_diag_noalloc": (C.c_int, [_vp, _dp]),
             "pic_species_sort": (C.c_int, [_vp]),
             "pic_species_inject_beam": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double,
                                                   C.c_double, C.c_uint64, C.c_uint64, _lp]),
@@ -332,6 +333,12 @@ class Species:
         """[real_count, px, py, pz, KE]"""
         out = np.zeros(5)
         _check(lib().pic_species_diag(self.h, _d(out)))
+        return out
+
+    def diag_noalloc(self):
+        """diag() through the entry point that allocates nothing (static device scratch)."""
+        out = np.zeros(5)
+        _check(lib().pic_species_diag_noalloc(self.h, _d(out)))
         return out
 
     def sort(self):
