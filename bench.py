@@ -65,6 +65,16 @@ def parse():
                          "bytes) that calibrates the HBM byte counters")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
                     help="particle form of --push-only")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiable sphere case (forward + adjoint gradient)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the differentiable sphere case and print its JSON")
+    ap.add_argument("--adjoint-grid", type=str, default="81,81,161")
+    ap.add_argument("--adjoint-particles", type=float, default=1e7)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--no-adjoint-fd", action="store_true",
+                    help="skip the finite-difference check of the adjoint gradient")
     return ap.parse_args()
 
 
@@ -285,6 +295,72 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     return out
 
 
+def adjoint_case(P, args):
+    """The differentiable sphere case: K steps forward in FP64 with soft absorption, then the
+    hand-written adjoint over checkpoints every M steps, for the loss
+    L = I/I0 + mean_wake(den^2)/D0 (ion current to the sphere plus the wake density L2,
+    normalised at theta0).  Optionally checked against central finite differences of the
+    forward run (eps = 1e-4 relative)."""
+    ni, nj, nk = (int(x) for x in args.adjoint_grid.split(","))
+    n_init = int(args.adjoint_particles)
+    vol = np.prod(np.subtract(Case.xm, Case.x0))
+    area = (Case.xm[0] - Case.x0[0]) * (Case.xm[1] - Case.x0[1])
+    # the beam's weight per macro-particle equals the fill's: J = v A dt n_init / V
+    n_inject = int(round(Case.v_drift * area * Case.dt * n_init / vol))
+    cfg = dict(ni=ni, nj=nj, nk=nk, x0=Case.x0, xm=Case.xm, sphere_c=Case.sphere_c,
+               sphere_r=Case.sphere_r, dt=Case.dt, steps=args.adjoint_steps,
+               checkpoint_every=args.adjoint_every, mass=16 * AMU, charge=QE, n0=Case.n0,
+               T_ion=args.T_ion, n_init=n_init, n_inject=n_inject, soft_width=1.0,
+               cg_tol=1e-6, nr_tol=1e-8, adj_rel_tol=1e-8, seed=1)
+    run = P.SphereAdjointRun(**cfg)
+    mask = run.wake_mask()
+    nn = ni * nj * nk
+    den = P.DeviceArray(nn)
+    theta = np.array([Case.v_drift, Case.phi_s, Case.Te])
+    cur = run.forward(theta, den.address)
+    d = den.get()
+    fwd_ms = run.stats()[0]
+    solves = run.stats()[2]
+    I0, D0 = cur, float(np.mean(d[mask] ** 2))
+    dden = P.DeviceArray.from_host(np.where(mask, 2 * d / (mask.sum() * D0), 0.0))
+    grad = run.backward(1.0 / I0, dden.address)
+    st = run.stats()
+    log(f"[bench] adjoint {ni}x{nj}x{nk}, {n_init:.3g} ions + {n_inject}/step, K={args.adjoint_steps}: "
+        f"forward {fwd_ms:.0f} ms, backward {st[1]:.0f} ms, dL/dln(theta) {grad * np.abs(theta)}")
+    slots = n_init + n_inject * (args.adjoint_steps + 1) / 2.0  # mean slots per step
+    out = {
+        "workload": f"differentiable sphere {ni}x{nj}x{nk}, {n_init:.3g} O+ (FP64) + {n_inject} "
+                    f"injected per step, K={args.adjoint_steps}, checkpoints every {args.adjoint_every}",
+        "theta": theta.tolist(), "loss": "I/I0 + mean_wake(den^2)/D0 at theta0",
+        "current_A": cur, "forward_ms": fwd_ms, "backward_ms": st[1],
+        "backward_over_forward": st[1] / fwd_ms,
+        "newton_solves_forward": solves, "adjoint_cg_iterations": st[3],
+        "forward_particle_steps_per_s": slots * args.adjoint_steps / (fwd_ms * 1e-3),
+        "grad": grad.tolist(), "dL_dln_theta": (grad * np.abs(theta)).tolist(),
+    }
+    if not args.no_adjoint_fd:
+        def L(t):
+            c_ = run.forward(t, den.address)
+            dd = den.get()
+            return c_ / I0 + float(np.mean(dd[mask] ** 2)) / D0
+        fd = np.zeros(3)
+        for a in range(3):
+            h = 1e-4 * abs(theta[a])
+            tp, tm = theta.copy(), theta.copy()
+            tp[a] += h
+            tm[a] -= h
+            fd[a] = (L(tp) - L(tm)) / (2 * h)
+        s_adj, s_fd = grad * np.abs(theta), fd * np.abs(theta)
+        out["fd_check"] = {"eps_rel": 1e-4, "fd_dL_dln_theta": s_fd.tolist(),
+                           "rel_err": (np.abs(s_adj - s_fd) / np.abs(s_fd)).tolist(),
+                           "err_over_max": (np.abs(s_adj - s_fd) / np.abs(s_fd).max()).tolist()}
+        log(f"[bench] adjoint fd check: {out['fd_check']}")
+    dden.close()
+    den.close()
+    run.close()
+    return out
+
+
 def cpu_baseline(args, grid, ef, T_ion):
     """Times the reference's own Species::advance + computeNumberDensity (oracle/_ref,
     built from the reference sources) on a bounded sample of the same workload: the same
@@ -336,6 +412,9 @@ def main():
     grid = tuple(int(x) for x in args.grid.split(","))
     if args.push_only:
         push_only(P, args, grid)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_case(P, args)}), flush=True)
         return
     horizon = args.horizon if args.horizon is not None else (0 if args.no_fp64 else 100)
     mixed = run_gpu(P, grid, args.particles, P.MIXED, args.steps, args.warmup, args.T_ion,
@@ -443,6 +522,8 @@ def main():
                 "steps": f"{mixed['steps_total'] - 19}-{mixed['steps_total']}",
                 "mixed": float(np.mean(mixed["late_push_ms"][-20:])),
                 "fp64": float(np.mean(fp64["late_push_ms"][-20:]))}
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_case(P, args)
     print(json.dumps(line), flush=True)
 
 

@@ -9,8 +9,8 @@ nodes on the GPU in the reference's flat order U = k*ni*nj + j*ni + i
 device memory and composes the loss.
 
 ``sphere_loss`` is the loss of the differentiable sphere case: the ion current to the sphere
-plus the L2 norm of the density in the wake behind it, each relative to its value at a
-reference theta.
+plus the L2 norm of the density in the wake behind it (``SphereAdjointRun.wake_mask``), each
+relative to its value at a reference theta.
 """
 from __future__ import annotations
 
@@ -44,17 +44,6 @@ class SphereRunFunction(torch.autograd.Function):
         grad = run.backward(gc, dden.data_ptr() if dden is not None else 0,
                             dphi.data_ptr() if dphi is not None else 0)
         return torch.tensor(grad, dtype=torch.float64), None
-
-
-def wake_mask(cfg, z0=0.22, z1=0.30, r=0.05):
-    """Nodes behind the sphere (cylinder of radius r about the axis, z in [z0, z1])."""
-    ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
-    dh = [(cfg.xm[a] - cfg.x0[a]) / (n - 1) for a, n in enumerate((ni, nj, nk))]
-    k, j, i = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(ni), indexing="ij")
-    x = cfg.x0[0] + dh[0] * i - cfg.sphere_c[0]
-    y = cfg.x0[1] + dh[1] * j - cfg.sphere_c[1]
-    z = cfg.x0[2] + dh[2] * k
-    return ((x * x + y * y <= r * r) & (z >= z0) & (z <= z1)).ravel()
 
 
 def sphere_loss(cur, den, mask, cur_ref, den_ref):

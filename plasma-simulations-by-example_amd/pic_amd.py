@@ -398,6 +398,70 @@ class PotentialSolver:
         _check(lib().pic_solver_compute_ef(self.h))
 
 
+class DeviceArray:
+    """n doubles of device memory from the HIP runtime libpicamd uses (hipMalloc through
+    ctypes), for callers of the entry points that take device pointers without torch."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            lib()  # the library's runtime is loaded first; bind that same file
+            # (torch, when imported, maps its own copy from its wheel: not that one)
+            path = "libamdhip64.so.7"
+            with open("/proc/self/maps") as f:
+                for line in f:
+                    if "libamdhip64.so" in line and "/" in line and "site-packages" not in line:
+                        path = line[line.index("/"):].strip()
+                        break
+            h = C.CDLL(path)
+            h.hipMalloc.argtypes = [C.POINTER(_vp), C.c_size_t]
+            h.hipFree.argtypes = [_vp]
+            h.hipMemcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = _vp()
+        rc = self.hip().hipMalloc(C.byref(self.ptr), 8 * max(self.n, 1))
+        if rc != 0:
+            raise PicError(f"hipMalloc of {self.n} doubles failed ({rc})")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        d = cls(a.size)
+        cls.hip().hipDeviceSynchronize()
+        if cls.hip().hipMemcpy(d.ptr, a.ctypes.data_as(_vp), 8 * a.size, 1) != 0:  # host to device
+            raise PicError("hipMemcpy to the device failed")
+        return d
+
+    def get(self):
+        out = np.empty(self.n)
+        self.hip().hipDeviceSynchronize()
+        if self.hip().hipMemcpy(out.ctypes.data_as(_vp), self.ptr, 8 * self.n, 2) != 0:  # device to host
+            raise PicError("hipMemcpy to the host failed")
+        return out
+
+    @property
+    def address(self):
+        return self.ptr.value or 0
+
+    def close(self):
+        if self.ptr and not _exiting:
+            self.hip().hipFree(self.ptr)
+        self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class SphereAdjointRun:
     """The differentiable flow past a sphere (pic_adjoint_*): K steps of the ch3/ver2 loop in
     FP64 with soft absorption, and their hand-written adjoint over checkpoints.
@@ -451,3 +515,14 @@ class SphereAdjointRun:
         out = np.zeros(4)
         _check(lib().pic_adjoint_stats(self.h, _d(out)))
         return out
+
+    def wake_mask(self, z0=0.22, z1=0.30, r=0.05):
+        """Nodes behind the sphere: the cylinder of radius r about its axis, z in [z0, z1]
+        (flat order U = k*ni*nj + j*ni + i)."""
+        c = self.cfg
+        dh = [(c.xm[a] - c.x0[a]) / (n - 1) for a, n in enumerate((c.ni, c.nj, c.nk))]
+        k, j, i = np.meshgrid(np.arange(c.nk), np.arange(c.nj), np.arange(c.ni), indexing="ij")
+        x = c.x0[0] + dh[0] * i - c.sphere_c[0]
+        y = c.x0[1] + dh[1] * j - c.sphere_c[1]
+        z = c.x0[2] + dh[2] * k
+        return ((x * x + y * y <= r * r) & (z >= z0) & (z <= z1)).ravel()

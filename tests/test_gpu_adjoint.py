@@ -39,15 +39,8 @@ def run():
 
 
 @pytest.fixture(scope="module")
-def mask():
-    import adjoint
-
-    class Cfg:
-        pass
-    c = Cfg()
-    for k, v in CASE.items():
-        setattr(c, k, v)
-    return adjoint.wake_mask(c)
+def mask(run):
+    return run.wake_mask()
 
 
 def gpu_outputs(run, theta):

@@ -228,28 +228,27 @@ def test_density_after_unfinished_push(prec, order):
     assert abs(np.sum(sp.den() * nv) / gw.sum() - 1) < (1e-12 if prec == P.FP64 else 1e-6)
 
 
-@pytest.mark.skip(reason="the 200k-particle jump does not overflow the staging list on the MI355X "
-                         "(no capacity error raised); the overflow case is being rebuilt")
 def test_staging_overflow_reported_at_next_call_without_getnp():
     """A push whose staging list fills up drops particles; the next advance reports
     PIC_ERR_CAPACITY (one call later, with no getNp in between)."""
-    n_nodes = (41, 9, 9)
+    n_nodes = (41, 9, 9)  # 5 x 1 x 1 tiles of 8^3 cells
     w = P.World(*n_nodes)
     w.set_extents((0.0, 0.0, 0.0), (0.4, 0.08, 0.08))
     w.set_field(P.EF, np.zeros(3 * np.prod(n_nodes)))
-    dh = 0.01
-    dt = 1e-7
-    n = 200000
+    dh, dt, n = 0.01, 1e-7, 200000
     rng = np.random.default_rng(2)
-    # every particle in tile 0, jumping 20 cells in +x in one step: into an empty tile two
-    # tiles away whose inflow room is tiny, and past the 65536-slot staging list
+    # every particle in tile 0, jumping 20 cells in +x in one step into tiles 2 and 3, whose
+    # inflow segments hold 64 each
     pos = np.column_stack([rng.uniform(0.001, 0.079, n), rng.uniform(0.001, 0.079, n),
                            rng.uniform(0.001, 0.079, n)])
     vel = np.tile([20 * dh / dt, 0.0, 0.0], (n, 1))
-    sp = P.Species(w, 16 * AMU, O.QE, 1.0, P.FP64, capacity=n)
-    sp.set_particles(pos, vel, np.ones(n))
-    sp.sort()
-    sp.advance(P.REFLECT, dt)  # drops particles; the error is not known yet
+    sp = P.Species(w, 16 * AMU, O.QE, 1.0, P.FP64, capacity=250000)
+    # four appends of 50000, each sorted into the bins, keep the staging list at its
+    # 65536-slot default (one upload of all 200000 would grow it to hold them)
+    for c in range(4):
+        sp.add_particles(pos[c::4], vel[c::4], np.ones(n // 4), dt)
+        sp.sort()
+    sp.advance(P.REFLECT, dt)  # ~134000 particles find no slot; the error is not known yet
     with pytest.raises(P.PicError, match="capacity"):
         sp.advance(P.REFLECT, dt)
 
