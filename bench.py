@@ -75,6 +75,16 @@ def parse():
     ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--no-adjoint-fd", action="store_true",
                     help="skip the finite-difference check of the adjoint gradient")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched ensemble of grounded boxes")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the batched ensemble and print its JSON")
+    ap.add_argument("--ensemble-boxes", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-scale", type=float, default=1.0,
+                    help="multiplies every box's particle counts (1: the 1e5-4e6 range)")
+    ap.add_argument("--no-worlds", action="store_true",
+                    help="skip timing the same boxes as separate worlds")
     return ap.parse_args()
 
 
@@ -343,21 +353,134 @@ def adjoint_case(P, args):
             c_ = run.forward(t, den.address)
             dd = den.get()
             return c_ / I0 + float(np.mean(dd[mask] ** 2)) / D0
-        fd = np.zeros(3)
-        for a in range(3):
-            h = 1e-4 * abs(theta[a])
-            tp, tm = theta.copy(), theta.copy()
-            tp[a] += h
-            tm[a] -= h
-            fd[a] = (L(tp) - L(tm)) / (2 * h)
-        s_adj, s_fd = grad * np.abs(theta), fd * np.abs(theta)
-        out["fd_check"] = {"eps_rel": 1e-4, "fd_dL_dln_theta": s_fd.tolist(),
-                           "rel_err": (np.abs(s_adj - s_fd) / np.abs(s_fd)).tolist(),
-                           "err_over_max": (np.abs(s_adj - s_fd) / np.abs(s_fd).max()).tolist()}
-        log(f"[bench] adjoint fd check: {out['fd_check']}")
+        s_adj = grad * np.abs(theta)
+        out["fd_check"] = {}
+        for eps in (1e-4, 1e-5):
+            fd = np.zeros(3)
+            for a in range(3):
+                h = eps * abs(theta[a])
+                tp, tm = theta.copy(), theta.copy()
+                tp[a] += h
+                tm[a] -= h
+                fd[a] = (L(tp) - L(tm)) / (2 * h)
+            s_fd = fd * np.abs(theta)
+            out["fd_check"][f"eps_rel_{eps:g}"] = {
+                "fd_dL_dln_theta": s_fd.tolist(),
+                "rel_err": (np.abs(s_adj - s_fd) / np.abs(s_fd)).tolist(),
+                "err_over_max": (np.abs(s_adj - s_fd) / np.abs(s_fd).max()).tolist()}
+            log(f"[bench] adjoint fd check eps {eps:g}: {out['fd_check'][f'eps_rel_{eps:g}']}")
     dden.close()
     den.close()
     run.close()
+    return out
+
+
+def ensemble_boxes(nbox, seed=1, scale=1.0):
+    """The batched workload: nbox grounded boxes (ch2/Main.cpp) drawn from a seed.  Nodes per
+    axis log-uniform in [17, 65] (17^3 to 65^3 nodes), particles per species log-uniform in
+    [1e5, 4e6], widths uniform in [0.05, 0.2] m, densities log-uniform in [1e10, 1e12] m^-3.
+    O+ ions fill the box, electrons its upper octant [0, L/2]^3, all at rest."""
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for _ in range(nbox):
+        n = int(round(np.exp(rng.uniform(np.log(17), np.log(65)))))
+        L = rng.uniform(0.05, 0.2)
+        dens = 10 ** rng.uniform(10, 12)
+        counts = [int(np.exp(rng.uniform(np.log(1e5), np.log(4e6))) * scale) for _ in range(2)]
+        boxes.append(dict(n=n, L=L, den=dens, counts=counts))
+    return boxes
+
+
+ENS_DT = 2e-10  # ch2/Main.cpp:19
+ENS_TOL = 1e-4  # ch2/Main.cpp:38
+
+
+def ensemble_case(P, args):
+    """Config-4 ensemble: every box's ch2 step (push + gather + deposit of both species,
+    density, rho, SOR, computeEF) in one hipGraph replay, against the same boxes run as
+    separate worlds through the per-world entry points."""
+    spec = [(16 * AMU, QE), (9.10938215e-31, -QE)]
+    bx = ensemble_boxes(args.ensemble_boxes, 1, args.ensemble_scale)
+    boxes = [((b["n"],) * 3, (-b["L"] / 2,) * 3, (b["L"] / 2,) * 3) for b in bx]
+    counts = [b["counts"] for b in bx]
+    ens = P.Ensemble(boxes, spec, counts, P.MIXED)
+    for i, b in enumerate(bx):
+        L = b["L"]
+        ens.load_box(i, 0, (-L / 2,) * 3, (L / 2,) * 3, b["den"], seed=1)
+        ens.load_box(i, 1, (0.0,) * 3, (L / 2,) * 3, b["den"], seed=1)
+    tuned = P.tuned_ensemble_params(ens)
+    ens.set_tuning(**tuned)
+    t0 = time.time()
+    ens.solve_initial(10000, ENS_TOL)
+    log(f"[bench] ensemble: {len(bx)} boxes, {ens.stats()['particles']:.3g} particles, "
+        f"{ens.stats()['nodes']:.3g} nodes; initial solve {time.time() - t0:.1f}s; tuning {tuned}")
+    ens.step(args.warmup, ENS_DT, 10000, ENS_TOL)
+    ens.step(args.ensemble_steps, ENS_DT, 10000, ENS_TOL)
+    st = ens.stats()
+    it, _ = ens.solver_status()
+    nparts = st["particles"]
+    out = {
+        "workload": f"{len(bx)} grounded boxes (seed 1), 17^3-65^3 nodes, 1e5-4e6 particles per "
+                    f"species x {args.ensemble_scale:g}, mixed precision, SOR tol {ENS_TOL}",
+        "particles": nparts, "nodes": st["nodes"], "ms_per_step": st["ms_per_step"],
+        "particle_steps_per_s": nparts / (st["ms_per_step"] * 1e-3),
+        "graph_kernels_per_step": st["graph_kernels"], "launches_per_step": 1,
+        "push_workgroups": st["push_workgroups"], "tuning": tuned,
+        "sor_iterations_last_step": {"max": int(np.max(np.abs(it))), "mean": float(np.mean(np.abs(it))),
+                                     "all_converged": bool((it > 0).all())},
+    }
+    log(f"[bench] ensemble: {st['ms_per_step']:.2f} ms/step, {out['particle_steps_per_s']:.3e} "
+        f"particle-steps/s, SOR its max {out['sor_iterations_last_step']['max']}")
+    ens.close()
+    if not args.no_worlds:
+        # the same boxes as separate worlds through the per-world entry points
+        worlds = []
+        for i, b in enumerate(bx):
+            L = b["L"]
+            w = P.World(b["n"], b["n"], b["n"])
+            w.set_extents((-L / 2,) * 3, (L / 2,) * 3)
+            sps = []
+            for s, (m, q) in enumerate(spec):
+                lo = (-L / 2,) * 3 if s == 0 else (0.0,) * 3
+                vol = L ** 3 if s == 0 else (L / 2) ** 3
+                sp = P.Species(w, m, q, b["den"] * vol / counts[i][s], P.MIXED,
+                               capacity=int(counts[i][s] * 1.05) + 1024)
+                sp.load_box_device(lo, (L / 2,) * 3, b["den"], counts[i][s], seed=1)
+                sps.append(sp)
+            solver = P.PotentialSolver(w, P.SOR_LINEAR, 10000, ENS_TOL)
+            for sp in sps:
+                sp.compute_number_density()
+            w.compute_charge_density(sps)
+            solver.solve()
+            solver.compute_ef()
+            worlds.append((w, sps, solver))
+        log("[bench] worlds: loaded")
+
+        def world_steps(k):
+            for _ in range(k):
+                for w, sps, solver in worlds:
+                    for sp in sps:
+                        sp.advance(P.REFLECT, ENS_DT)
+                    for sp in sps:
+                        sp.compute_number_density()
+                    w.compute_charge_density(sps)
+                    solver.solve()
+                    solver.compute_ef()
+            for w, _, _ in worlds:
+                w.sync()
+        world_steps(args.warmup)
+        k = max(2, args.ensemble_steps // 4)
+        t0 = time.perf_counter()
+        world_steps(k)
+        wms = 1e3 * (time.perf_counter() - t0) / k
+        out["separate_worlds"] = {"ms_per_step": wms, "particle_steps_per_s": nparts / (wms * 1e-3),
+                                  "ensemble_speedup": wms / st["ms_per_step"]}
+        log(f"[bench] separate worlds: {wms:.2f} ms/step ({wms / st['ms_per_step']:.2f}x the ensemble)")
+        for w, sps, solver in worlds:
+            solver.close()
+            for sp in sps:
+                sp.close()
+            w.close()
     return out
 
 
@@ -415,6 +538,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_case(P, args)}), flush=True)
+        return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_case(P, args)}), flush=True)
         return
     horizon = args.horizon if args.horizon is not None else (0 if args.no_fp64 else 100)
     mixed = run_gpu(P, grid, args.particles, P.MIXED, args.steps, args.warmup, args.T_ion,
@@ -524,6 +650,8 @@ def main():
                 "fp64": float(np.mean(fp64["late_push_ms"][-20:]))}
     if not args.no_adjoint:
         line["adjoint"] = adjoint_case(P, args)
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_case(P, args)
     print(json.dumps(line), flush=True)
 
 

@@ -29,8 +29,9 @@
 extern "C" {
 #endif
 
-/* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL); 3: pic_solver_get_info; 4: pic_adjoint_* */
-#define PICAMD_ABI_VERSION 4
+/* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL); 3: pic_solver_get_info; 4: pic_adjoint_*;
+ * 5: pic_ensemble_* */
+#define PICAMD_ABI_VERSION 5
 
 /* status codes */
 #define PIC_OK 0
@@ -258,6 +259,59 @@ int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, co
 /* out[4]: last forward ms, last backward ms, Newton solves of the last forward, CG iterations
  * of the last backward's linear solves. */
 int pic_adjoint_stats(pic_adjoint *a, double out[4]);
+
+/* ---- Batched ensemble of grounded boxes ----------------------------------------------------
+ * No counterpart in the reference: many independent ch2 worlds (ch2/Main.cpp:14-75) of uneven
+ * size stepped together.  Each box has specular walls (ch2/Species.cpp:32-36), linear SOR
+ * w = 1.4 with the residual every 25 iterations (ch2/PotentialSolver.cpp:11-67) and
+ * computeEF (:70-110).  One launch per phase covers every box through a box-descriptor
+ * table, and the step's launches are captured once in a hipGraph and replayed.  Particle
+ * counts per (box, species) are fixed at creation (the walls reflect: none is lost). */
+typedef struct pic_ensemble pic_ensemble;
+typedef struct {
+    int ni, nj, nk;       /* nodes */
+    double x0[3], xm[3];  /* World::setExtents */
+} pic_box;
+/* nbox boxes, nsp species (mass, charge per species), np[b*nsp + s] particles of species s
+ * in box b; precision PIC_MIXED or PIC_FP64; track_ids: carry a uint32 id per particle. */
+int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *mass,
+                        const double *charge, const long *np, int precision, int track_ids,
+                        pic_ensemble **out);
+int pic_ensemble_destroy(pic_ensemble *e);
+/* Host upload of all n particles of (box b, species s) (n must equal the count at creation;
+ * FP64 interleaved positions and velocities, no rewind); ids become 0..n-1. */
+int pic_ensemble_set_particles(pic_ensemble *e, int b, int s, long n, const double *pos,
+                               const double *vel, const double *mpw);
+/* Species::loadParticlesBox (ch2/Species.cpp:74-97) on the device: the range's particles
+ * uniform in [x1, x2) at rest, weight num_den*V/n; Philox draws keyed by (seed, b, s). */
+int pic_ensemble_load_box(pic_ensemble *e, int b, int s, const double x1[3], const double x2[3],
+                          double num_den, uint64_t seed);
+int pic_ensemble_get_particles(pic_ensemble *e, int b, int s, double *pos, double *vel,
+                               double *mpw, uint32_t *ids);
+/* Counting sort of every (box, species) range by cell. */
+int pic_ensemble_sort(pic_ensemble *e);
+/* Tile and launch shapes (the autotuner's parameters): particles per push workgroup, LDS
+ * deposit window in nodes (0: global atomics only; at most 8192), threads per SOR
+ * workgroup (256, 512, 1024), steps between counting sorts (0: never); use_graph 0 launches
+ * each step's kernels eagerly instead of replaying the captured graph. */
+int pic_ensemble_set_tuning(pic_ensemble *e, int chunk_particles, int window_nodes,
+                            int sor_threads, int sort_every, int use_graph);
+/* The initial solve of every box before the loop (ch2/Main.cpp:38-42): density, rho, SOR,
+ * computeEF with the particles where they stand. */
+int pic_ensemble_solve_initial(pic_ensemble *e, int max_it, double tol);
+/* nsteps steps of advance -> computeNumberDensity -> computeChargeDensity -> solve ->
+ * computeEF for every box (ch2/Main.cpp:45-61), one hipGraph replay per step. */
+int pic_ensemble_step(pic_ensemble *e, int nsteps, double dt, int max_it, double tol);
+/* Host copy of box b's field: 0 phi, 1 rho, 2 E (3 per node), 10+s density of species s. */
+int pic_ensemble_get_field(pic_ensemble *e, int b, int which, double *out);
+/* Per box: SOR iterations of the last solve (negative when it did not converge) and the last
+ * residual norm (nullable arrays of nbox). */
+int pic_ensemble_solver_status(pic_ensemble *e, int *iters, double *residual);
+/* out[5*(b*nsp+s) + q]: sum w, sum w vx, sum w vy, sum w vz, sum w |v|^2 of each range. */
+int pic_ensemble_diag(pic_ensemble *e, double *out);
+/* out[5]: ms per step of the last pic_ensemble_step, kernels in the step's graph, particles,
+ * nodes, push workgroups. */
+int pic_ensemble_stats(pic_ensemble *e, double out[5]);
 
 #ifdef __cplusplus
 }

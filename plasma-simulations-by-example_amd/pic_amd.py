@@ -41,7 +41,18 @@ EXPORTS = [
     "pic_solver_solve", "pic_solver_get_info", "pic_solver_solve_qn", "pic_solver_compute_ef",
     "pic_adjoint_create", "pic_adjoint_destroy", "pic_adjoint_forward", "pic_adjoint_backward",
     "pic_adjoint_stats",
+    "pic_ensemble_create", "pic_ensemble_destroy", "pic_ensemble_set_particles",
+    "pic_ensemble_load_box", "pic_ensemble_get_particles", "pic_ensemble_sort",
+    "pic_ensemble_set_tuning", "pic_ensemble_solve_initial", "pic_ensemble_step",
+    "pic_ensemble_get_field", "pic_ensemble_solver_status", "pic_ensemble_diag",
+    "pic_ensemble_stats",
 ]
+
+
+class Box(C.Structure):
+    """pic_box: nodes per axis and extents of one grounded box."""
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+                ("x0", C.c_double * 3), ("xm", C.c_double * 3)]
 
 
 class AdjointConfig(C.Structure):
@@ -140,6 +151,24 @@ def lib():
             "pic_adjoint_forward": (C.c_int, [_vp, _dp, _dp, _vp, _vp]),
             "pic_adjoint_backward": (C.c_int, [_vp, C.c_double, _vp, _vp, _dp]),
             "pic_adjoint_stats": (C.c_int, [_vp, _dp]),
+            "pic_ensemble_create": (C.c_int, [C.c_int, C.POINTER(Box), C.c_int, _dp, _dp, _lp,
+                                              C.c_int, C.c_int, C.POINTER(_vp)]),
+            "pic_ensemble_destroy": (C.c_int, [_vp]),
+            "pic_ensemble_set_particles": (C.c_int, [_vp, C.c_int, C.c_int, C.c_long, _dp, _dp,
+                                                     _dp]),
+            "pic_ensemble_load_box": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                                C.c_uint64]),
+            "pic_ensemble_get_particles": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp,
+                                                     _u32p]),
+            "pic_ensemble_sort": (C.c_int, [_vp]),
+            "pic_ensemble_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                  C.c_int]),
+            "pic_ensemble_solve_initial": (C.c_int, [_vp, C.c_int, C.c_double]),
+            "pic_ensemble_step": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_double]),
+            "pic_ensemble_get_field": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+            "pic_ensemble_solver_status": (C.c_int, [_vp, C.POINTER(C.c_int), _dp]),
+            "pic_ensemble_diag": (C.c_int, [_vp, _dp]),
+            "pic_ensemble_stats": (C.c_int, [_vp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -526,3 +555,112 @@ class SphereAdjointRun:
         y = c.x0[1] + dh[1] * j - c.sphere_c[1]
         z = c.x0[2] + dh[2] * k
         return ((x * x + y * y <= r * r) & (z >= z0) & (z <= z1)).ravel()
+
+
+class Ensemble:
+    """Many independent ch2 grounded boxes stepped together (pic_ensemble_*): one launch per
+    phase over a box-descriptor table, each step one hipGraph replay.
+
+    boxes: list of (nodes (ni, nj, nk), x0, xm); species: list of (mass, charge);
+    counts[b][s]: particles of species s in box b."""
+
+    PHI, RHO, EF = 0, 1, 2
+
+    def __init__(self, boxes, species, counts, precision=MIXED, track_ids=False):
+        nb, ns = len(boxes), len(species)
+        arr = (Box * nb)()
+        self.nodes = []
+        for b, (n, x0, xm) in enumerate(boxes):
+            arr[b].ni, arr[b].nj, arr[b].nk = (int(t) for t in n)
+            arr[b].x0[:] = [float(t) for t in x0]
+            arr[b].xm[:] = [float(t) for t in xm]
+            self.nodes.append(tuple(int(t) for t in n))
+        mass = np.ascontiguousarray([m for m, _ in species], dtype=np.float64)
+        charge = np.ascontiguousarray([q for _, q in species], dtype=np.float64)
+        cnt = np.ascontiguousarray(np.asarray(counts, dtype=np.int64).reshape(nb * ns))
+        self.nbox, self.nsp = nb, ns
+        self.counts = cnt.reshape(nb, ns)
+        self.h = _vp()
+        _check(lib().pic_ensemble_create(nb, arr, ns, _d(mass), _d(charge),
+                                         cnt.ctypes.data_as(_lp), precision, int(track_ids),
+                                         C.byref(self.h)))
+
+    def close(self):
+        if self.h and not _exiting:
+            lib().pic_ensemble_destroy(self.h)
+        self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nn(self, b):
+        ni, nj, nk = self.nodes[b]
+        return ni * nj * nk
+
+    def set_particles(self, b, s, pos, vel, mpw):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        mpw = np.ascontiguousarray(mpw, dtype=np.float64)
+        _check(lib().pic_ensemble_set_particles(self.h, b, s, len(mpw), _d(pos), _d(vel), _d(mpw)))
+
+    def load_box(self, b, s, x1, x2, num_den, seed=1):
+        _check(lib().pic_ensemble_load_box(self.h, b, s, _d(_v3(x1)), _d(_v3(x2)), num_den, seed))
+
+    def particles(self, b, s, ids=False):
+        n = int(self.counts[b, s])
+        pos, vel, mpw = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        idv = np.zeros(n, dtype=np.uint32) if ids else None
+        _check(lib().pic_ensemble_get_particles(self.h, b, s, _d(pos), _d(vel), _d(mpw),
+                                                idv.ctypes.data_as(_u32p) if ids else None))
+        return (pos, vel, mpw, idv) if ids else (pos, vel, mpw)
+
+    def sort(self):
+        _check(lib().pic_ensemble_sort(self.h))
+
+    def set_tuning(self, chunk_particles=4096, window_nodes=6144, sor_threads=1024, sort_every=20,
+                   use_graph=True):
+        _check(lib().pic_ensemble_set_tuning(self.h, chunk_particles, window_nodes, sor_threads,
+                                             sort_every, int(use_graph)))
+
+    def solve_initial(self, max_it=10000, tol=1e-4):
+        _check(lib().pic_ensemble_solve_initial(self.h, max_it, tol))
+
+    def step(self, nsteps, dt, max_it=10000, tol=1e-4):
+        _check(lib().pic_ensemble_step(self.h, nsteps, dt, max_it, tol))
+
+    def field(self, b, which):
+        n = self.nn(b) * (3 if which == self.EF else 1)
+        out = np.zeros(n)
+        _check(lib().pic_ensemble_get_field(self.h, b, which, _d(out)))
+        return out
+
+    def den(self, b, s):
+        out = np.zeros(self.nn(b))
+        _check(lib().pic_ensemble_get_field(self.h, b, 10 + s, _d(out)))
+        return out
+
+    def solver_status(self):
+        it = (C.c_int * self.nbox)()
+        res = np.zeros(self.nbox)
+        _check(lib().pic_ensemble_solver_status(self.h, it, _d(res)))
+        return np.array(it[:]), res
+
+    def diag(self):
+        out = np.zeros(5 * self.nbox * self.nsp)
+        _check(lib().pic_ensemble_diag(self.h, _d(out)))
+        return out.reshape(self.nbox, self.nsp, 5)
+
+    def stats(self):
+        """{"ms_per_step", "graph_kernels", "particles", "nodes", "push_workgroups"}"""
+        out = np.zeros(5)
+        _check(lib().pic_ensemble_stats(self.h, _d(out)))
+        return {"ms_per_step": out[0], "graph_kernels": int(out[1]), "particles": int(out[2]),
+                "nodes": int(out[3]), "push_workgroups": int(out[4])}
+
+
+def tuned_ensemble_params(ens):
+    """Launch shapes for an ensemble: the defaults until a tuning table is present."""
+    return dict(chunk_particles=4096, window_nodes=6144, sor_threads=1024, sort_every=20)
