@@ -11,11 +11,14 @@
 //     atomic per non-zero node; deposits that fall outside the window go straight to global
 //     atomics;
 //   - k_ens_finish: den = raw / node_vol per species (clearing raw), rho = sum q_s den_s;
-//   - k_ens_sor: one workgroup per box runs the box's whole SOR solve (sweeps, residual,
-//     stopping test) with phi in LDS when the box fits, in global memory otherwise;
+//   - k_ens_sor: one workgroup per box whose phi fits in LDS runs that box's whole SOR solve
+//     (sweeps, residual, stopping test); the larger boxes are swept by one launch per
+//     half-sweep over all of them (k_ens_sweep), chunks of 25 iterations and a per-box
+//     residual check (k_ens_resid, k_ens_check), converged boxes skipping at once;
 //   - k_ens_ef: computeEF of every box.
-// The step's four launches are captured once in a hipGraph and replayed; a counting sort of
-// every range runs between replays every sort_every steps.
+// The step is captured in three hipGraphs (push + density + the first SOR iteration, a chunk
+// of 25 SOR iterations, computeEF) and replayed; the host reads the count of converged boxes
+// between chunks.  A counting sort of every range runs between steps every sort_every steps.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -445,6 +448,123 @@ __global__ void __launch_bounds__(1024) k_ens_sor(EnsSorArgs S) {
     }
 }
 
+// ---- SOR of the boxes that do not fit in LDS: one launch per half-sweep over all of them ----
+// The same iteration as k_ens_sor, spread over the whole GPU: every box gets its own run of
+// workgroups (a workgroup never spans two boxes), a box that has converged skips the launch
+// at once (its workgroups return), and the residual of iteration 0, 25, 50, ... is reduced per
+// box in a fixed order.
+struct EnsSweepArgs {
+    const EnsBox *box;
+    const int *blk_box;  // box of each sweep workgroup
+    const long *blk0;    // first sweep workgroup of each box
+    double *phi;
+    const double *rho;
+    const int *done;
+    double w;
+};
+
+__global__ void __launch_bounds__(256) k_ens_sweep(EnsSweepArgs S, int color) {
+    const int bi = S.blk_box[blockIdx.x];
+    if (S.done[bi]) return;
+    const EnsBox B = S.box[bi];
+    const unsigned mi = (unsigned)(B.ni - 2), mj = (unsigned)(B.nj - 2);
+    const unsigned half = (mi + 1) / 2, rows = mj * (unsigned)(B.nk - 2);
+    const unsigned q = (unsigned)(blockIdx.x - S.blk0[bi]) * 256u + threadIdx.x;
+    if (q >= half * rows) return;
+    unsigned r = q / half, sl = q - r * half;
+    unsigned j = 1 + r % mj, k = 1 + r / mj;
+    unsigned i = 1 + 2 * sl + (((1 + j + k) & 1) != (unsigned)color ? 1u : 0u);
+    if (i > mi) return;
+    const long sj = B.ni, sk = (long)B.ni * B.nj;
+    double *phi = S.phi + B.node0;
+    const double *rho = S.rho + B.node0;
+    const double idx2 = 1.0 / (B.dh[0] * B.dh[0]), idy2 = 1.0 / (B.dh[1] * B.dh[1]),
+                 idz2 = 1.0 / (B.dh[2] * B.dh[2]);
+    const double denom = 2 * idx2 + 2 * idy2 + 2 * idz2;
+    long u = i + sj * j + sk * k;
+    double phi_new = (rho[u] / EPS_0 + idx2 * (phi[u - 1] + phi[u + 1]) + idy2 * (phi[u - sj] + phi[u + sj]) +
+                      idz2 * (phi[u - sk] + phi[u + sk])) /
+                     denom;
+    phi[u] = phi[u] + S.w * (phi_new - phi[u]);
+}
+
+// per-workgroup partial sums of R^2 over the interior nodes (ch2/PotentialSolver.cpp:48-58)
+__global__ void __launch_bounds__(256) k_ens_resid(EnsSweepArgs S, double *part) {
+    __shared__ double sh[4];
+    const int bi = S.blk_box[blockIdx.x];
+    if (S.done[bi]) {
+        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
+    const EnsBox B = S.box[bi];
+    const unsigned mi = (unsigned)(B.ni - 2), mj = (unsigned)(B.nj - 2);
+    const unsigned rows = mj * (unsigned)(B.nk - 2);
+    const unsigned q = (unsigned)(blockIdx.x - S.blk0[bi]) * 256u + threadIdx.x;
+    double R2 = 0.0;
+    if (q < mi * rows) {
+        unsigned r = q / mi;
+        unsigned i = 1 + (q - r * mi);
+        unsigned j = 1 + r % mj, k = 1 + r / mj;
+        const long sj = B.ni, sk = (long)B.ni * B.nj;
+        const double *phi = S.phi + B.node0;
+        const double *rho = S.rho + B.node0;
+        const double idx2 = 1.0 / (B.dh[0] * B.dh[0]), idy2 = 1.0 / (B.dh[1] * B.dh[1]),
+                     idz2 = 1.0 / (B.dh[2] * B.dh[2]);
+        const double denom = 2 * idx2 + 2 * idy2 + 2 * idz2;
+        long u = i + sj * j + sk * k;
+        double R = -phi[u] * denom + rho[u] / EPS_0 + idx2 * (phi[u - 1] + phi[u + 1]) +
+                   idy2 * (phi[u - sj] + phi[u + sj]) + idz2 * (phi[u - sk] + phi[u + sk]);
+        R2 = R * R;
+    }
+    double t = block_sum(R2, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+struct EnsCheckArgs {
+    const EnsBox *box;
+    const int *large;   // the boxes of this path
+    const long *rblk0;  // first residual workgroup of each box
+    const long *rblkn;  // residual workgroups of each box
+    const double *part;
+    double tol;
+    int *box_it, *done, *iters, *ndone;
+    double *res;
+};
+
+// one workgroup per box: L2 = sqrt(sum R^2 / (ni nj nk)) after `its` more iterations; the box
+// stops when L2 < tol (its iteration count then is the reference's it + 1)
+__global__ void __launch_bounds__(256) k_ens_check(EnsCheckArgs C, int its) {
+    __shared__ double sh[4];
+    const int bi = C.large[blockIdx.x];
+    if (C.done[bi]) return;
+    const EnsBox B = C.box[bi];
+    double s = 0.0;
+    for (long q = threadIdx.x; q < C.rblkn[bi]; q += 256) s += C.part[C.rblk0[bi] + q];
+    double tot = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        double L2 = sqrt(tot / (double)(B.ni * B.nj * B.nk));
+        int it = C.box_it[bi] + its;
+        C.box_it[bi] = it;
+        C.res[bi] = L2;
+        if (L2 < C.tol) {
+            C.done[bi] = 1;
+            C.iters[bi] = it;
+            atomicAdd(C.ndone, 1);
+        } else {
+            C.iters[bi] = -it;
+        }
+    }
+}
+
+__global__ void k_ens_sor_reset(int n, const int *large, int *box_it, int *done, int *ndone) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
+        box_it[large[q]] = 0;
+        done[large[q]] = 0;
+    }
+    if (q == 0) *ndone = 0;
+}
+
 // ---- counting sort of every range by cell ---------------------------------------------------
 struct EnsSortArgs {
     const EnsBox *box;
@@ -681,6 +801,16 @@ struct pic_ensemble {
     double *res = nullptr;
     int *sor_small = nullptr, *sor_large = nullptr;  // box lists
     int n_small = 0, n_large = 0;
+    // the boxes whose phi is not held in LDS: sweep and residual workgroup tables, per-box
+    // iteration counters and flags, the count of converged boxes (and its pinned mirror)
+    int *sw_blk_box = nullptr, *rs_blk_box = nullptr;
+    long *sw_blk0 = nullptr, *rs_blk0 = nullptr, *rs_blkn = nullptr;
+    long n_sw_blk = 0, n_rs_blk = 0;
+    double *rs_part = nullptr;
+    int *box_it = nullptr, *done = nullptr, *ndone = nullptr, *ndone_host = nullptr;
+    hipGraph_t graph_c = nullptr, graph_e = nullptr;
+    hipGraphExec_t exec_c = nullptr, exec_e = nullptr;
+    long last_sor_its = 0;  // iterations issued to the large boxes in the last step
     size_t sor_lds = 0;  // bytes of LDS for the small-box SOR
     uint32_t *cnt = nullptr, *off = nullptr;  // sort: per-cell counts, then their exclusive scan
     void *scan_tmp = nullptr;
@@ -735,10 +865,14 @@ int build_chunks(pic_ensemble *e) {
 }
 
 int drop_graph(pic_ensemble *e) {
-    if (e->exec) hipGraphExecDestroy(e->exec);
-    if (e->graph) hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
+    hipGraphExec_t *x[3] = {&e->exec, &e->exec_c, &e->exec_e};
+    hipGraph_t *g[3] = {&e->graph, &e->graph_c, &e->graph_e};
+    for (int i = 0; i < 3; i++) {
+        if (*x[i]) hipGraphExecDestroy(*x[i]);
+        if (*g[i]) hipGraphDestroy(*g[i]);
+        *x[i] = nullptr;
+        *g[i] = nullptr;
+    }
     return PIC_OK;
 }
 
@@ -779,7 +913,9 @@ int launch_finish(pic_ensemble *e) {
     return PIC_OK;
 }
 
-int launch_sor(pic_ensemble *e, int max_it, double tol) {
+// the boxes held in LDS: one workgroup each runs the whole solve
+int launch_sor_small(pic_ensemble *e, int max_it, double tol) {
+    if (!e->n_small) return PIC_OK;
     EnsSorArgs S;
     S.box = e->box;
     S.phi = e->phi;
@@ -789,16 +925,48 @@ int launch_sor(pic_ensemble *e, int max_it, double tol) {
     S.max_it = max_it;
     S.iters = e->iters;
     S.res = e->res;
-    if (e->n_small) {
-        S.boxes = e->sor_small;
-        k_ens_sor<true><<<e->n_small, e->sor_threads, e->sor_lds, e->stream>>>(S);
-    }
-    if (e->n_large) {
-        S.boxes = e->sor_large;
-        k_ens_sor<false><<<e->n_large, e->sor_threads, 0, e->stream>>>(S);
-    }
+    S.boxes = e->sor_small;
+    k_ens_sor<true><<<e->n_small, e->sor_threads, e->sor_lds, e->stream>>>(S);
     PIC_LAUNCH_CHECK();
     return PIC_OK;
+}
+
+EnsSweepArgs sweep_args(pic_ensemble *e, bool resid) {
+    EnsSweepArgs S;
+    S.box = e->box;
+    S.blk_box = resid ? e->rs_blk_box : e->sw_blk_box;
+    S.blk0 = resid ? e->rs_blk0 : e->sw_blk0;
+    S.phi = e->phi;
+    S.rho = e->rho;
+    S.done = e->done;
+    S.w = 1.4;  // ch2/PotentialSolver.cpp:40
+    return S;
+}
+
+// `its` iterations of every large box not yet converged, then its residual check
+int launch_sor_chunk(pic_ensemble *e, int its, double tol) {
+    if (!e->n_large) return PIC_OK;
+    EnsSweepArgs SW = sweep_args(e, false), RS = sweep_args(e, true);
+    for (int it = 0; it < its; it++)
+        for (int color = 0; color < 2; color++)
+            k_ens_sweep<<<(unsigned)e->n_sw_blk, 256, 0, e->stream>>>(SW, color);
+    k_ens_resid<<<(unsigned)e->n_rs_blk, 256, 0, e->stream>>>(RS, e->rs_part);
+    EnsCheckArgs C{e->box, e->sor_large, e->rs_blk0, e->rs_blkn, e->rs_part, tol,
+                   e->box_it, e->done, e->iters, e->ndone, e->res};
+    k_ens_check<<<(unsigned)e->n_large, 256, 0, e->stream>>>(C, its);
+    PIC_LAUNCH_CHECK();
+    return PIC_OK;
+}
+
+int launch_sor_start(pic_ensemble *e, int max_it, double tol) {
+    int rc = launch_sor_small(e, max_it, tol);
+    if (rc) return rc;
+    if (!e->n_large) return PIC_OK;
+    k_ens_sor_reset<<<(e->n_large + 255) / 256, 256, 0, e->stream>>>(e->n_large, e->sor_large, e->box_it, e->done,
+                                                                    e->ndone);
+    PIC_LAUNCH_CHECK();
+    // iteration 0 and the reference's first residual check (it % 25 == 0 at it = 0)
+    return launch_sor_chunk(e, 1, tol);
 }
 
 int launch_ef(pic_ensemble *e) {
@@ -849,38 +1017,89 @@ int do_sort(pic_ensemble *e) {
     return PIC_OK;
 }
 
-// the four launches of one step
-int enqueue_step(pic_ensemble *e, double dt, int max_it, double tol) {
+// The step in three parts: (1) push, density and rho, the SOR of the LDS boxes and the first
+// iteration of the others; (2) chunks of 25 iterations of the boxes not yet converged,
+// issued until all have converged or max_it iterations have run; (3) computeEF.
+// CHUNK: the reference checks the residual every 25 iterations (ch2/PotentialSolver.cpp:46).
+constexpr int SOR_CHUNK = 25;
+
+int enqueue_first(pic_ensemble *e, double dt, int max_it, double tol) {
     int rc = launch_push(e, dt);
     if (rc) return rc;
     rc = launch_finish(e);
     if (rc) return rc;
-    rc = launch_sor(e, max_it, tol);
-    if (rc) return rc;
-    return launch_ef(e);
+    return launch_sor_start(e, max_it, tol);
+}
+
+int capture(pic_ensemble *e, int part, double dt, int max_it, double tol, hipGraph_t *g, hipGraphExec_t *x,
+            int *nodes) {
+    PIC_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = part == 0 ? enqueue_first(e, dt, max_it, tol) : part == 1 ? launch_sor_chunk(e, SOR_CHUNK, tol) : launch_ef(e);
+    hipGraph_t gg = nullptr;
+    hipError_t ee = hipStreamEndCapture(e->stream, &gg);
+    if (rc) {
+        if (gg) hipGraphDestroy(gg);
+        return rc;
+    }
+    if (ee != hipSuccess) return hip_fail(ee, "hipStreamEndCapture", __FILE__, __LINE__);
+    *g = gg;
+    size_t nn = 0;
+    PIC_HIP(hipGraphGetNodes(gg, nullptr, &nn));
+    *nodes += (int)nn;
+    PIC_HIP(hipGraphInstantiate(x, gg, nullptr, nullptr, 0));
+    return PIC_OK;
 }
 
 int ensure_graph(pic_ensemble *e, double dt, int max_it, double tol) {
     if (e->exec && e->g_dt == dt && e->g_max_it == max_it && e->g_tol == tol) return PIC_OK;
     drop_graph(e);
-    PIC_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(e, dt, max_it, tol);
-    hipGraph_t g = nullptr;
-    hipError_t ee = hipStreamEndCapture(e->stream, &g);
+    e->graph_nodes = 0;
+    int rc = capture(e, 0, dt, max_it, tol, &e->graph, &e->exec, &e->graph_nodes);
+    if (!rc && e->n_large) rc = capture(e, 1, dt, max_it, tol, &e->graph_c, &e->exec_c, &e->graph_nodes);
+    if (!rc) rc = capture(e, 2, dt, max_it, tol, &e->graph_e, &e->exec_e, &e->graph_nodes);
     if (rc) {
-        if (g) hipGraphDestroy(g);
+        drop_graph(e);
         return rc;
     }
-    if (ee != hipSuccess) return hip_fail(ee, "hipStreamEndCapture", __FILE__, __LINE__);
-    e->graph = g;
-    size_t nn = 0;
-    PIC_HIP(hipGraphGetNodes(g, nullptr, &nn));
-    e->graph_nodes = (int)nn;
-    PIC_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
     e->g_dt = dt;
     e->g_max_it = max_it;
     e->g_tol = tol;
     return PIC_OK;
+}
+
+// one whole step, from the graphs or eagerly
+int run_step(pic_ensemble *e, double dt, int max_it, double tol) {
+    int rc;
+    if (e->use_graph) {
+        PIC_HIP(hipGraphLaunch(e->exec, e->stream));
+    } else if ((rc = enqueue_first(e, dt, max_it, tol))) {
+        return rc;
+    }
+    long its = 1;
+    if (e->n_large) {
+        // chunks between host reads of the converged count: 1, 2, 4, ... (at most 8)
+        int burst = 1;
+        for (;;) {
+            PIC_HIP(hipMemcpyAsync(e->ndone_host, e->ndone, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+            PIC_HIP(hipStreamSynchronize(e->stream));
+            if (*e->ndone_host >= e->n_large || its >= max_it) break;
+            for (int b = 0; b < burst && its < max_it; b++) {
+                if (e->use_graph) {
+                    PIC_HIP(hipGraphLaunch(e->exec_c, e->stream));
+                } else if ((rc = launch_sor_chunk(e, SOR_CHUNK, tol))) {
+                    return rc;
+                }
+                its += SOR_CHUNK;
+            }
+            burst = std::min(2 * burst, 8);
+        }
+    }
+    e->last_sor_its = its;
+    if (e->use_graph) {
+        PIC_HIP(hipGraphLaunch(e->exec_e, e->stream));
+        return PIC_OK;
+    }
+    return launch_ef(e);
 }
 
 int box_range_check(pic_ensemble *e, int b, int s) {
@@ -992,6 +1211,44 @@ int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *m
     if ((rc = ens_alloc((void **)&e->blk0, sizeof(long) * b0.size()))) return fail(rc);
     PIC_HIP(hipMemcpy(e->blk_box, bb.data(), sizeof(int) * bb.size(), hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(e->blk0, b0.data(), sizeof(long) * b0.size(), hipMemcpyHostToDevice));
+    {
+        // sweep and residual workgroups of the boxes outside LDS, box by box
+        std::vector<int> swb, rsb;
+        std::vector<long> sw0((size_t)nbox, 0), rs0((size_t)nbox, 0), rsn((size_t)nbox, 0);
+        for (int b : large) {
+            const EnsBox &B = e->hbox[b];
+            long mi = B.ni - 2, rows = (long)(B.nj - 2) * (B.nk - 2);
+            long per_color = (mi + 1) / 2 * rows;
+            sw0[b] = (long)swb.size();
+            for (long q = 0; q < (per_color + 255) / 256; q++) swb.push_back(b);
+            rs0[b] = (long)rsb.size();
+            rsn[b] = (mi * rows + 255) / 256;
+            for (long q = 0; q < rsn[b]; q++) rsb.push_back(b);
+        }
+        e->n_sw_blk = (long)swb.size();
+        e->n_rs_blk = (long)rsb.size();
+        int **ip[2] = {&e->sw_blk_box, &e->rs_blk_box};
+        std::vector<int> *iv[2] = {&swb, &rsb};
+        for (int k = 0; k < 2; k++) {
+            if ((rc = ens_alloc((void **)ip[k], sizeof(int) * std::max<size_t>(iv[k]->size(), 1)))) return fail(rc);
+            if (!iv[k]->empty())
+                PIC_HIP(hipMemcpy(*ip[k], iv[k]->data(), sizeof(int) * iv[k]->size(), hipMemcpyHostToDevice));
+        }
+        long **lp[3] = {&e->sw_blk0, &e->rs_blk0, &e->rs_blkn};
+        std::vector<long> *lv[3] = {&sw0, &rs0, &rsn};
+        for (int k = 0; k < 3; k++) {
+            if ((rc = ens_alloc((void **)lp[k], sizeof(long) * (size_t)nbox))) return fail(rc);
+            PIC_HIP(hipMemcpy(*lp[k], lv[k]->data(), sizeof(long) * nbox, hipMemcpyHostToDevice));
+        }
+        if ((rc = ens_alloc((void **)&e->rs_part, sizeof(double) * std::max<long>(e->n_rs_blk, 1)))) return fail(rc);
+        int **fl[3] = {&e->box_it, &e->done, &e->ndone};
+        for (int k = 0; k < 3; k++) {
+            if ((rc = ens_alloc((void **)fl[k], sizeof(int) * (size_t)(nbox + 1)))) return fail(rc);
+            PIC_HIP(hipMemset(*fl[k], 0, sizeof(int) * (size_t)(nbox + 1)));
+        }
+        hipError_t he = hipHostMalloc((void **)&e->ndone_host, sizeof(int), hipHostMallocDefault);
+        if (he != hipSuccess) return fail(hip_fail(he, "hipHostMalloc", __FILE__, __LINE__));
+    }
     if ((rc = ens_alloc((void **)&e->sor_small, sizeof(int) * std::max<size_t>(small.size(), 1)))) return fail(rc);
     if ((rc = ens_alloc((void **)&e->sor_large, sizeof(int) * std::max<size_t>(large.size(), 1)))) return fail(rc);
     if (!small.empty()) PIC_HIP(hipMemcpy(e->sor_small, small.data(), sizeof(int) * small.size(), hipMemcpyHostToDevice));
@@ -1054,9 +1311,11 @@ int pic_ensemble_destroy(pic_ensemble *e) {
     drop_graph(e);
     void *ptrs[] = {e->box, e->rng, e->chunk, e->blk_box, e->blk0, e->pmem[0], e->pmem[1], e->raw, e->den, e->rho,
                     e->phi, e->charges_dev, e->ef32, e->ef64, e->iters, e->res, e->sor_small, e->sor_large, e->cnt,
-                    e->off, e->scan_tmp, e->diag_part};
+                    e->off, e->scan_tmp, e->diag_part, e->sw_blk_box, e->rs_blk_box, e->sw_blk0, e->rs_blk0,
+                    e->rs_blkn, e->rs_part, e->box_it, e->done, e->ndone};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (e->ndone_host) hipHostFree(e->ndone_host);
     if (e->ev0) hipEventDestroy(e->ev0);
     if (e->ev1) hipEventDestroy(e->ev1);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -1171,9 +1430,12 @@ int pic_ensemble_solve_initial(pic_ensemble *e, int max_it, double tol) {
     }
     PIC_HIP(hipMemcpyAsync(e->rng, e->hrng.data(), sizeof(EnsRange) * e->hrng.size(), hipMemcpyHostToDevice,
                            e->stream));
-    int rc = enqueue_step(e, 0.0, max_it, tol);
-    if (rc) return rc;
     drop_graph(e);
+    bool g = e->use_graph;
+    e->use_graph = false;
+    int rc = run_step(e, 0.0, max_it, tol);
+    e->use_graph = g;
+    if (rc) return rc;
     e->g_dt = -1;
     PIC_HIP(hipStreamSynchronize(e->stream));
     return PIC_OK;
@@ -1208,12 +1470,8 @@ int pic_ensemble_step(pic_ensemble *e, int nsteps, double dt, int max_it, double
             rc = do_sort(e);
             if (rc) return rc;
         }
-        if (e->use_graph) {
-            PIC_HIP(hipGraphLaunch(e->exec, e->stream));
-        } else {
-            rc = enqueue_step(e, dt, max_it, tol);
-            if (rc) return rc;
-        }
+        rc = run_step(e, dt, max_it, tol);
+        if (rc) return rc;
         e->steps_done++;
         e->steps_since_sort++;
     }
