@@ -30,6 +30,8 @@ for _p in (ROOT, PKG):
 
 import numpy as np  # noqa: E402
 
+from workloads import Case, ensemble_boxes, initial_potential  # noqa: E402
+
 QE = 1.602176565e-19
 AMU = 1.660538921e-27
 K_B = 1.380648e-23
@@ -161,36 +163,6 @@ def copy_bandwidth():
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-class Case:
-    x0 = (-0.1, -0.1, 0.0)
-    xm = (0.1, 0.1, 0.4)
-    sphere_c = (0.0, 0.0, 0.15)
-    sphere_r = 0.05
-    phi_s = -100.0
-    n0 = 1e10
-    v_drift = 7000.0
-    Te = 1.5
-    dt = 1e-7
-
-
-def initial_potential(P, w, solver_tol=1e-2):
-    """Initial phi for a domain preloaded with a uniform plasma.  Undamped Newton from
-    phi = 0 around a -100 V sphere diverges (the exp(phi/Te) term overshoots); the
-    reference never meets that state because its domain fills from vacuum.  Continuation
-    in the sphere potential (-25, -50, -75, -100 V) with a loose-tolerance Newton solve reaches
-    the sheath solution; the timed steps then use the reference solver settings."""
-    s = P.PotentialSolver(w, P.MG, 5000, solver_tol)
-    s.set_reference_values(0.0, Case.Te, Case.n0)
-    for k in range(1, 5):
-        phi_s = Case.phi_s * k / 4
-        w.add_sphere(Case.sphere_c, Case.sphere_r, phi_s)
-        t0 = time.time()
-        ok, res, its = s.solve()
-        log(f"[bench]   continuation phi_s={phi_s:.0f} V: converged {ok}, {its} CG its, "
-            f"{time.time() - t0:.1f}s")
-    s.close()
 
 
 def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=False, phi=None, horizon=0):
@@ -375,22 +347,6 @@ def adjoint_case(P, args):
     return out
 
 
-def ensemble_boxes(nbox, seed=1, scale=1.0):
-    """The batched workload: nbox grounded boxes (ch2/Main.cpp) drawn from a seed.  Nodes per
-    axis log-uniform in [17, 65] (17^3 to 65^3 nodes), particles per species log-uniform in
-    [1e5, 4e6], widths uniform in [0.05, 0.2] m, densities log-uniform in [1e10, 1e12] m^-3.
-    O+ ions fill the box, electrons its upper octant [0, L/2]^3, all at rest."""
-    rng = np.random.default_rng(seed)
-    boxes = []
-    for _ in range(nbox):
-        n = int(round(np.exp(rng.uniform(np.log(17), np.log(65)))))
-        L = rng.uniform(0.05, 0.2)
-        dens = 10 ** rng.uniform(10, 12)
-        counts = [int(np.exp(rng.uniform(np.log(1e5), np.log(4e6))) * scale) for _ in range(2)]
-        boxes.append(dict(n=n, L=L, den=dens, counts=counts))
-    return boxes
-
-
 ENS_DT = 2e-10  # ch2/Main.cpp:19
 ENS_TOL = 1e-4  # ch2/Main.cpp:38
 
@@ -408,8 +364,7 @@ def ensemble_case(P, args):
         L = b["L"]
         ens.load_box(i, 0, (-L / 2,) * 3, (L / 2,) * 3, b["den"], seed=1)
         ens.load_box(i, 1, (0.0,) * 3, (L / 2,) * 3, b["den"], seed=1)
-    tuned = P.tuned_ensemble_params(ens)
-    ens.set_tuning(**tuned)
+    tuned = P.tuned_ensemble_params(ens)  # applied by the library at creation
     t0 = time.time()
     ens.solve_initial(10000, ENS_TOL)
     log(f"[bench] ensemble: {len(bx)} boxes, {ens.stats()['particles']:.3g} particles, "

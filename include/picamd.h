@@ -30,8 +30,8 @@ extern "C" {
 #endif
 
 /* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL); 3: pic_solver_get_info; 4: pic_adjoint_*;
- * 5: pic_ensemble_* */
-#define PICAMD_ABI_VERSION 5
+ * 5: pic_ensemble_*; 6: tuning table, pic_species_set_launch, pic_solver_set_mg */
+#define PICAMD_ABI_VERSION 6
 
 /* status codes */
 #define PIC_OK 0
@@ -174,6 +174,10 @@ int pic_species_diag(pic_species *s, double out[5]);
 int pic_species_diag_noalloc(pic_species *s, double out[5]);
 /* Counting sort of all particles into tile bins (tile-major cell order). */
 int pic_species_sort(pic_species *s);
+/* No counterpart in the reference (launch shape of the push, the autotuner's parameters):
+ * particles per push workgroup and push-map entries per XCD stripe.  Species take both from
+ * the tuning table at creation (else 32768 and 32). */
+int pic_species_set_launch(pic_species *s, long push_chunk, int xcd_stripe);
 /* ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27) when T == 0, WarmBeamSource::sample
  * (ch4/Source.cpp:31-56) when T > 0 [K], on the device: num = (int)(den*v*A*dt/mpw0 + U)
  * particles uniform on the k=0 face, v = v_drift z (+ isotropic thermal speed), then the
@@ -206,6 +210,11 @@ int pic_solver_solve(pic_solver *s, double *residual, int *iters);
  * from PCG to solveGSLinear (ch3/ver2/PotentialSolver.cpp:270-271), info[3] 1 when it
  * converged. */
 int pic_solver_get_info(pic_solver *s, int info[4]);
+/* No counterpart in the reference (launch shapes of PIC_SOLVER_MG, the autotuner's
+ * parameters): smoothing sweep pairs on the 7-point levels, first level with Galerkin
+ * operators, largest level of the single-workgroup coarse cycle (nodes); 0 keeps the tuning
+ * table's value (else the built-in default).  The hierarchy is rebuilt by the next solve. */
+int pic_solver_set_mg(pic_solver *s, int sweeps, int gal_from, long fuse_max);
 /* solveQN() with the solver's reference values (ch3/ver2/PotentialSolver.cpp:204-222). */
 int pic_solver_solve_qn(pic_solver *s);
 /* computeEF() (ch3/ver2/PotentialSolver.cpp:465-504). */
@@ -259,6 +268,19 @@ int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, co
 /* out[4]: last forward ms, last backward ms, Newton solves of the last forward, CG iterations
  * of the last backward's linear solves. */
 int pic_adjoint_stats(pic_adjoint *a, double out[4]);
+
+/* ---- Tuning table --------------------------------------------------------------------------
+ * No counterpart in the reference.  Launch shapes measured by the autotuner
+ * (plasma-simulations-by-example_amd/autotune.py), one entry per line:
+ * "<kind> <nodes> <particles> key=value ...", kinds push, mg and ensemble.  Creation of a
+ * species, a multigrid hierarchy or an ensemble takes the entry of its kind nearest to its
+ * size; without one the built-in defaults apply.  The table is read from PICAMD_TUNING, else
+ * tuning/table.txt beside libpicamd.so, on first use. */
+/* (Re)loads the table from path (null: the default location). */
+int pic_tuning_load(const char *path);
+/* The value of key for kind at (nodes, particles), dflt when the table has none. */
+int pic_tuning_lookup(const char *kind, double nodes, double particles, const char *key,
+                      double dflt, double *value);
 
 /* ---- Batched ensemble of grounded boxes ----------------------------------------------------
  * No counterpart in the reference: many independent ch2 worlds (ch2/Main.cpp:14-75) of uneven

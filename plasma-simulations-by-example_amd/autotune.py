@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Autotuner of the library's launch shapes.
+
+Measures candidate shapes on the GPU through the C ABI and writes the fastest to the tuning
+table (tuning/table.txt beside libpicamd.so), which the library reads at run time:
+
+  push      particles per push workgroup ("chunk") and push-map entries per XCD stripe
+            ("stripe"): pushes of the sphere workload (gather + leapfrog + deposit + rebin),
+            timed by the push kernel's HIP events;
+  mg        smoothing sweep pairs ("sweeps"), first Galerkin level ("gal_from") and largest
+            level of the single-workgroup coarse cycle ("fuse_max") of the multigrid
+            preconditioner: one Newton + MG-CG solve of the sphere potential;
+  ensemble  particles per push workgroup ("chunk"), LDS deposit window ("window"), SOR
+            workgroup size ("sor_threads") and steps between counting sorts ("sort_every") of
+            the batched grounded boxes: whole ensemble steps.
+
+The search is coordinate descent from the built-in defaults: one parameter at a time over its
+candidates with the others held, the best kept, two passes.  Every table line records the
+size it was tuned at; a lookup takes the nearest entry of its kind.
+
+    python autotune.py --target push --grid 257,257,513 --particles 1e9
+    python autotune.py --target mg --grid 257,257,513
+    python autotune.py --target ensemble --boxes 64
+"""
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+import pic_amd as P  # noqa: E402
+from workloads import Case, ensemble_boxes, initial_potential, log  # noqa: E402
+
+QE = 1.602176565e-19
+AMU = 1.660538921e-27
+ME = 9.10938215e-31
+
+SPACE = {
+    "push": {"chunk": [16384, 32768, 65536, 131072], "stripe": [8, 16, 32, 64]},
+    "mg": {"sweeps": [1, 2], "gal_from": [1, 2, 3], "fuse_max": [300, 2000, 12000]},
+    "ensemble": {"chunk": [1024, 2048, 4096, 8192, 16384], "window": [0, 2048, 4096, 6144, 8192],
+                 "sor_threads": [256, 512, 1024], "sort_every": [5, 10, 20, 50]},
+}
+DEFAULTS = {
+    "push": {"chunk": 32768, "stripe": 32},
+    "mg": {"sweeps": 1, "gal_from": 2, "fuse_max": 2000},
+    "ensemble": {"chunk": 4096, "window": 6144, "sor_threads": 1024, "sort_every": 20},
+}
+
+
+def coordinate_descent(kind, measure, passes=2):
+    """Minimises measure(params) one parameter at a time; returns (best, ms, trials)."""
+    best = dict(DEFAULTS[kind])
+    best_ms = measure(best)
+    trials = [(dict(best), best_ms)]
+    log(f"[tune] {kind} start {best}: {best_ms:.3f} ms")
+    for _ in range(passes):
+        for key, values in SPACE[kind].items():
+            for v in values:
+                if v == best[key]:
+                    continue
+                cand = dict(best)
+                cand[key] = v
+                ms = measure(cand)
+                trials.append((cand, ms))
+                log(f"[tune] {kind} {cand}: {ms:.3f} ms")
+                if ms < best_ms:
+                    best, best_ms = cand, ms
+    # the winner measured again beside the default, so that a noisy first reading of either
+    # does not decide
+    again = measure(best)
+    dflt = measure(DEFAULTS[kind])
+    if dflt <= again:
+        best, best_ms = dict(DEFAULTS[kind]), dflt
+    else:
+        best_ms = again
+    return best, best_ms, trials
+
+
+def sphere_world(grid):
+    w = P.World(*grid)
+    w.set_extents(Case.x0, Case.xm)
+    w.add_sphere(Case.sphere_c, Case.sphere_r, Case.phi_s)
+    w.add_inlet()
+    return w
+
+
+def tune_push(args):
+    """Each measurement starts from the same state: the particles reloaded from the same seed
+    into the field of the initial potential, then 1 + steps pushes, the median push-kernel
+    time of the timed ones."""
+    grid = tuple(int(x) for x in args.grid.split(","))
+    w = sphere_world(grid)
+    n = int(args.particles)
+    vol = np.prod(np.subtract(Case.xm, Case.x0))
+    ions = P.Species(w, 16 * AMU, QE, Case.n0 * vol / n, P.MIXED, capacity=int(n * 1.02))
+
+    def load():
+        # replace every particle (a device load appends): empty the species, then load
+        ions.set_particles(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))
+        ions.load_box_device(Case.x0, Case.xm, Case.n0, n, Case.v_drift, 1000.0, seed=1)
+
+    load()
+    ions.compute_number_density()
+    w.compute_charge_density([ions])
+    initial_potential(P, w)
+    s = P.PotentialSolver(w, P.QN, 1, 1e-4)
+    s.compute_ef()
+    w.set_timing(True)
+
+    def measure(p):
+        ions.set_launch(p["chunk"], p["stripe"])
+        load()
+        ions.advance(P.ABSORB, Case.dt)
+        ions.compute_number_density()
+        w.sync()
+        w.timers(reset=True)
+        ts = []
+        for _ in range(args.steps):
+            ions.advance(P.ABSORB, Case.dt)
+            ions.compute_number_density()
+            w.sync()
+            t = w.timers(reset=True)["push_kernel"]
+            ts.append(t[0] / max(t[1], 1))
+        return float(np.median(ts))
+
+    res = coordinate_descent("push", measure, passes=1)
+    return grid, n, res
+
+
+def tune_mg(args):
+    grid = tuple(int(x) for x in args.grid.split(","))
+    w = sphere_world(grid)
+    n = int(args.particles)
+    vol = np.prod(np.subtract(Case.xm, Case.x0))
+    ions = P.Species(w, 16 * AMU, QE, Case.n0 * vol / n, P.MIXED, capacity=int(n * 1.02))
+    ions.load_box_device(Case.x0, Case.xm, Case.n0, n, Case.v_drift, 1000.0, seed=1)
+    ions.compute_number_density()
+    w.compute_charge_density([ions])
+    initial_potential(P, w)
+    phi_start = w.field(P.PHI)
+    # the solve measured: one push moves the ions, then Newton + MG-CG with the reference's
+    # tolerances from the potential before it (a time step's solve)
+    s0 = P.PotentialSolver(w, P.QN, 1, 1e-4)
+    s0.compute_ef()
+    ions.advance(P.ABSORB, Case.dt)
+    ions.compute_number_density()
+    w.compute_charge_density([ions])
+    solver = P.PotentialSolver(w, P.MG, 10000, 1e-4)
+    solver.set_reference_values(0.0, Case.Te, Case.n0)
+
+    def measure(p):
+        solver.set_mg(p["sweeps"], p["gal_from"], p["fuse_max"])
+        times = []
+        for rep in range(args.steps):
+            w.set_field(P.PHI, phi_start)
+            w.sync()
+            t0 = time.perf_counter()
+            ok, _, _ = solver.solve()
+            w.sync()
+            if not ok:
+                return float("inf")
+            if rep > 0 or args.steps == 1:  # the first solve builds the hierarchy
+                times.append(1e3 * (time.perf_counter() - t0))
+        return float(np.median(times))
+
+    res = coordinate_descent("mg", measure)
+    return grid, 0, res
+
+
+def tune_ensemble(args):
+    """Each measurement starts from the same state: every box reloaded from the same seeds and
+    solved, then warmup + steps whole ensemble steps."""
+    spec = [(16 * AMU, QE), (ME, -QE)]
+    bx = ensemble_boxes(args.boxes, 1, args.scale)
+    boxes = [((b["n"],) * 3, (-b["L"] / 2,) * 3, (b["L"] / 2,) * 3) for b in bx]
+    ens = P.Ensemble(boxes, spec, [b["counts"] for b in bx], P.MIXED)
+    dt, tol = 2e-10, 1e-4  # ch2/Main.cpp:19,38
+
+    def reset():
+        for i, b in enumerate(bx):
+            L = b["L"]
+            ens.load_box(i, 0, (-L / 2,) * 3, (L / 2,) * 3, b["den"], seed=1)
+            ens.load_box(i, 1, (0.0,) * 3, (L / 2,) * 3, b["den"], seed=1)
+        ens.solve_initial(10000, tol)
+
+    def measure(p):
+        ens.set_tuning(chunk_particles=p["chunk"], window_nodes=p["window"], sor_threads=p["sor_threads"],
+                       sort_every=p["sort_every"])
+        reset()
+        ens.step(args.warmup, dt, 10000, tol)
+        ens.step(args.steps, dt, 10000, tol)
+        return ens.stats()["ms_per_step"]
+
+    st = ens.stats()
+    res = coordinate_descent("ensemble", measure)
+    return (st["nodes"],), st["particles"], res
+
+
+def write_table(path, kind, nodes, particles, params, ms, note):
+    """Replaces the entry of kind at this size, keeps the others."""
+    lines = []
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if f and not line.startswith("#") and f[0] == kind and float(f[1]) == float(nodes) \
+                    and float(f[2]) == float(particles):
+                continue
+            lines.append(line.rstrip("\n"))
+    if not any(line.startswith("#") for line in lines):
+        lines.insert(0, "# kind nodes particles key=value ...  (written by autotune.py; read by libpicamd)")
+    kv = " ".join(f"{k}={v}" for k, v in params.items())
+    lines.append(f"{kind} {int(nodes)} {int(particles)} {kv}  # {ms:.3f} ms; {note}")
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--target", choices=("push", "mg", "ensemble"), required=True)
+    ap.add_argument("--grid", default="257,257,513")
+    ap.add_argument("--particles", type=float, default=1e9)
+    ap.add_argument("--boxes", type=int, default=64)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--table", default=os.path.join(HERE, "tuning", "table.txt"))
+    args = ap.parse_args()
+    if args.target == "push":
+        grid, np_, (best, ms, trials) = tune_push(args)
+        nodes = int(np.prod(grid))
+        note = f"push kernel, sphere {'x'.join(map(str, grid))}, {np_:.3g} particles"
+    elif args.target == "mg":
+        grid, np_, (best, ms, trials) = tune_mg(args)
+        nodes = int(np.prod(grid))
+        note = f"Newton + MG-CG solve, sphere {'x'.join(map(str, grid))}"
+    else:
+        (nodes,), np_, (best, ms, trials) = tune_ensemble(args)
+        note = f"ensemble step, {args.boxes} boxes"
+    write_table(args.table, args.target, nodes, np_, best, ms, note)
+    print({"target": args.target, "best": best, "ms": ms, "trials": len(trials), "nodes": nodes,
+           "particles": np_, "table": args.table}, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1299,7 +1299,16 @@ int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *m
         e->scan_tmp_bytes = bytes;
         if ((rc = ens_alloc(&e->scan_tmp, bytes))) return fail(rc);
     }
-    // per-range dt factors are set by pic_ensemble_step; chunks for the default chunk size
+    // launch shapes for this size from the tuning table (else the defaults); per-range dt
+    // factors are set by pic_ensemble_step
+    {
+        double nd = (double)e->total_nodes, pd = (double)std::max<long>(e->np, 1);
+        e->chunk_particles = std::max(256, (int)tuned("ensemble", nd, pd, "chunk", e->chunk_particles));
+        e->window_nodes = std::min(8192, std::max(0, (int)tuned("ensemble", nd, pd, "window", e->window_nodes)));
+        int st = (int)tuned("ensemble", nd, pd, "sor_threads", e->sor_threads);
+        e->sor_threads = (st == 256 || st == 512 || st == 1024) ? st : 1024;
+        e->sort_every = std::max(0, (int)tuned("ensemble", nd, pd, "sort_every", e->sort_every));
+    }
     if ((rc = build_chunks(e))) return fail(rc);
     *out = e;
     return PIC_OK;

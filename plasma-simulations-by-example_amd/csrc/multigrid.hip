@@ -80,6 +80,12 @@ constexpr float MG_JACOBI_W = MG_JAC_W;
 
 struct MgHierarchy {
     std::vector<MgLevel> L;
+    // launch shapes (MG_NU, MG_GAL_FROM, MG_FUSE_MAX unless the tuning table or the solver
+    // sets others): smoothing sweep pairs on the 7-point levels, first Galerkin level, largest
+    // level of the single-workgroup coarse cycle
+    int sweeps = MG_NU;
+    int gal_from = MG_GAL_FROM;
+    long fuse_max = 2000;
     unsigned rap_stamp = ~0u;  // world geometry the Galerkin operators were built for
     int coarse_sweeps = 0;
     int fuse_l0 = -1;       // first level of the single-workgroup coarse cycle (-1: none)
@@ -663,6 +669,8 @@ struct MgDev {
 #define MG_FUSE_MAX 2000
 #endif
 constexpr long MG_FUSE_NODES = MG_FUSE_MAX;  // about 2 nodes per thread and step
+// the largest level a tuning table or a solver may hand to the single workgroup
+constexpr long MG_FUSE_NODES_MAX = 100000;
 constexpr int MG_FUSE_THREADS = 1024;
 
 __global__ void __launch_bounds__(MG_FUSE_THREADS)
@@ -788,6 +796,13 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
     if (*out) return PIC_OK;
     pic_world *w = s->w;
     MgHierarchy *H = new MgHierarchy();
+    H->fuse_max = MG_FUSE_NODES;
+    double nodes = (double)w->g.nn;
+    H->sweeps = s->mg_sweeps > 0 ? s->mg_sweeps : (int)tuned("mg", nodes, 0.0, "sweeps", MG_SWEEPS);
+    H->gal_from = s->mg_gal_from > 0 ? s->mg_gal_from : (int)tuned("mg", nodes, 0.0, "gal_from", MG_GALERKIN_FROM);
+    H->fuse_max = s->mg_fuse_max > 0 ? s->mg_fuse_max : (long)tuned("mg", nodes, 0.0, "fuse_max", (double)MG_FUSE_NODES);
+    H->sweeps = std::max(1, H->sweeps);
+    H->gal_from = std::max(1, H->gal_from);
     MgLevel l0;
     l0.g = w->g;
     l0.oid = w->object_id;
@@ -804,7 +819,7 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
         c.g.nn = (long)c.g.ni * c.g.nj * c.g.nk;
         for (int a = 0; a < 3; a++) c.g.dh[a] = 2 * f.dh[a];
         c.owned = true;
-        c.gal = (int)H->L.size() >= MG_GALERKIN_FROM;
+        c.gal = (int)H->L.size() >= H->gal_from;
         H->L.push_back(c);
     }
     hipError_t e = hipSuccess;
@@ -839,7 +854,7 @@ int mg_build(pic_solver *s, MgHierarchy **out) {
     // residual, so the fused cycle starts at level 1 or deeper)
     const int nl = (int)H->L.size();
     for (int l = 1; l < nl; l++)
-        if (H->L[l].g.nn <= MG_FUSE_NODES) {
+        if (H->L[l].g.nn <= H->fuse_max && H->L[l].g.nn <= MG_FUSE_NODES_MAX) {
             H->fuse_l0 = l;
             break;
         }
@@ -915,7 +930,7 @@ static void down(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevSc
         return;
     }
     k_mg_pre<TB><<<nbk8(f.g.nn), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, sc);
-    for (int it = 1; it < MG_SWEEPS; it++) {  // further (red, black) pairs
+    for (int it = 1; it < H->sweeps; it++) {  // further (red, black) pairs
         k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
         k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
     }
@@ -948,7 +963,7 @@ static void up(MgHierarchy *H, int l, const TB *b, hipStream_t st, const DevScal
     }
     k_mg_prolong<<<smooth_blocks(f.g), 256, 0, st>>>(f.g, c.g, f.mask, c.x, f.x, sc);
     k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
-    for (int it = 1; it < MG_SWEEPS; it++) {  // the pre-smoothing pairs in reverse: (black, red)
+    for (int it = 1; it < H->sweeps; it++) {  // the pre-smoothing pairs in reverse: (black, red)
         k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 0, sc);
         k_mg_smooth<TB><<<smooth_blocks(f.g), 256, 0, st>>>(f.g, C, f.mask, f.idiag, f.x, b, 1, sc);
     }

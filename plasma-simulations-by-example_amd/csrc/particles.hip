@@ -637,17 +637,17 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_starts(Grid g, PlanTmp T,
     if (t == 0) *out.stage_count = 0;
 }
 
-// push workgroups per tile: one per PUSH_CHUNK of its input particles, at least one
+// push workgroups per tile: one per `chunk` of its input particles, at least one
 // and which instantiation pushes each tile: the wide-rim one when at least 16 of its
 // particles, and 1/128 of them, deposited outside the window in the last push (force: 1 all
 // tiles, -1 none); the far counts restart from 0
 __global__ void __launch_bounds__(PLAN_THREADS)
-    k_plan_chunks(Grid g, PlanTmp T, uint32_t *far, uint32_t *wide, int force) {
+    k_plan_chunks(Grid g, PlanTmp T, uint32_t *far, uint32_t *wide, int force, uint32_t chunk) {
     __shared__ unsigned long long sh[PLAN_THREADS / 64];
     int t = blockIdx.x * PLAN_THREADS + threadIdx.x;
     uint32_t c = 0;
     if (t < g.ntiles) {
-        c = max(1u, (T.need[t] + PUSH_CHUNK - 1) / PUSH_CHUNK);
+        c = max(1u, (T.need[t] + chunk - 1) / chunk);
         T.nchunk[t] = c;
         uint32_t f = far[t];
         far[t] = 0;
@@ -695,6 +695,8 @@ struct PushOut {
     double *diag_part;  // [NDIAG_SLOTS][gridDim.x] per-block partials, reduced on demand
     uint32_t *err;
     uint32_t *far;          // per tile: particles deposited outside the window this push
+    uint32_t chunk;         // particles per push workgroup (the plan's chunk size)
+    uint32_t stripe;        // map entries per XCD stripe
 };
 
 // slot of an arrival in the inflow segment of tile dtile, -1 when the segment is full
@@ -765,12 +767,12 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         // workgroup -> (tile, chunk) from the plan's map.  Workgroups are dispatched
         // round-robin over the 8 XCDs (XCD b % 8, PUSH_STAGE_WGS a multiple of 8); the
         // j-th binned one of an XCD takes entry (j / S * 8 + xcd) * S + j % S,
-        // S = PUSH_XCD_STRIPE: a bijection of [0, nbin) (nbin is a multiple of 8 S) that
+        // S = O.stripe: a bijection of [0, nbin) (nbin is a multiple of 8 S) that
         // deals out stripes of S entries round-robin
         const unsigned total = (unsigned)min(*O.total_groups, (unsigned long long)nbin);
         const unsigned b = blockIdx.x - PUSH_STAGE_WGS;
         const unsigned xcd = b % 8u, j = b / 8u;
-        const unsigned wg = ((j / PUSH_XCD_STRIPE) * 8u + xcd) * PUSH_XCD_STRIPE + j % PUSH_XCD_STRIPE;
+        const unsigned wg = ((j / O.stripe) * 8u + xcd) * O.stripe + j % O.stripe;
         if (wg >= total) {
             if (tid < NDIAG_SLOTS) O.diag_part[tid * gridDim.x + blockIdx.x] = 0.0;
             return;
@@ -781,9 +783,9 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         if (((tc >> 63) != 0) != WIDE) return;  // the other instantiation's tile
         BinSeg bs = bin_of(in, t);
         const uint32_t cnt_tile = bs.c0 + bs.c1;
-        const bool split = cnt_tile > (uint32_t)PUSH_CHUNK;  // several workgroups share tile t
-        const uint32_t q_lo = chunk * (uint32_t)PUSH_CHUNK;
-        const uint32_t q_hi = min(cnt_tile, q_lo + (uint32_t)PUSH_CHUNK);
+        const bool split = cnt_tile > O.chunk;  // several workgroups share tile t
+        const uint32_t q_lo = chunk * O.chunk;
+        const uint32_t q_hi = min(cnt_tile, q_lo + O.chunk);
         if (q_lo >= q_hi) {
             if (tid == 0) O.win_used[t] = 0;
             if (tid < NDIAG_SLOTS) O.diag_part[tid * gridDim.x + blockIdx.x] = 0.0;
@@ -1598,18 +1600,19 @@ static size_t plan_bytes(const Grid &g) {
     return (((12 + 4 * (size_t)PLAN_STAGED_SHARDS) * (size_t)g.ntiles + 7) & ~(size_t)7) + 16 + 8 + 24 * nblk;
 }
 
-// push workgroups over the tiles: at most one per tile plus one per PUSH_CHUNK bin slots,
+// push workgroups over the tiles: at most one per tile plus one per chunk of bin slots,
 // padded to whole rounds of XCD stripes
-static long push_bin_groups(const Grid &g, long cap_bins) {
-    const long r = 8L * PUSH_XCD_STRIPE;
-    long n = (long)g.ntiles + cap_bins / PUSH_CHUNK + 1;
+static long push_bin_groups(const pic_species *s, long cap_bins) {
+    const Grid &g = s->w->g;
+    const long r = 8L * s->xcd_stripe;
+    long n = (long)g.ntiles + cap_bins / s->push_chunk + 1;
     return (n + r - 1) / r * r;
 }
 
 // chunk map and per-workgroup diagnostic partials sized for the buffers' bin capacity
 static int ensure_push_scratch(pic_species *s) {
     const Grid &g = s->w->g;
-    long nb = push_bin_groups(g, s->buf[1 - s->cur].cap_bins);
+    long nb = push_bin_groups(s, s->buf[1 - s->cur].cap_bins);
     if (nb > s->chunk_cap) {
         if (s->chunk_map) PIC_HIP(hipFree(s->chunk_map));
         s->chunk_map = nullptr;
@@ -1649,7 +1652,8 @@ static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist,
         // PIC_PUSH_WIDE=all / off forces the wide-rim push for every tile / for none (tests)
         const char *fw = getenv("PIC_PUSH_WIDE");
         const int force = !fw ? 0 : !strcmp(fw, "all") ? 1 : !strcmp(fw, "off") ? -1 : 0;
-        k_plan_chunks<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->far_tile, s->far_tile + g.ntiles, force);
+        k_plan_chunks<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->far_tile, s->far_tile + g.ntiles, force,
+                                                            (uint32_t)s->push_chunk);
         k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk2, nblk, T.total);
         k_plan_map<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->far_tile + g.ntiles, s->chunk_map, s->chunk_cap);
     }
@@ -1816,6 +1820,11 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     s->charge = charge;
     s->mpw0 = mpw0;
     s->prec = precision;
+    // launch shapes of the push for this grid and particle count (tuning table, else defaults)
+    s->push_chunk = std::max<long>(256, (long)tuned("push", (double)w->g.nn, (double)std::max<long>(capacity, 1),
+                                                      "chunk", (double)PUSH_CHUNK));
+    s->xcd_stripe = std::max(1, (int)tuned("push", (double)w->g.nn, (double)std::max<long>(capacity, 1), "stripe",
+                                           (double)PUSH_XCD_STRIPE));
     s->track_ids = track_ids != 0;
     const Grid &g = w->g;
     long cap = std::max<long>(capacity, 1024);
@@ -2141,8 +2150,10 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     O.diag_part = s->diag_part;
     O.err = err_flag(s);
     O.far = s->far_tile;
+    O.chunk = (uint32_t)s->push_chunk;
+    O.stripe = (uint32_t)s->xcd_stripe;
     // workgroups for the tiles (the plan's map decides which run) + the staging workgroups
-    unsigned grid = (unsigned)(push_bin_groups(g, out.cap_bins) + PUSH_STAGE_WGS);
+    unsigned grid = (unsigned)(push_bin_groups(s, out.cap_bins) + PUSH_STAGE_WGS);
     s->diag_nb = (int)grid;
     hipEvent_t evk;
     timer_begin(w, PIC_T_PUSH_KERNEL, &evk);
@@ -2260,6 +2271,16 @@ int pic_species_diag(pic_species *s, double out[5]) {
     out[2] = s->diag_host[2] * s->mass;
     out[3] = s->diag_host[3] * s->mass;
     out[4] = 0.5 * s->mass * s->diag_host[4];
+    return PIC_OK;
+}
+
+int pic_species_set_launch(pic_species *s, long push_chunk, int xcd_stripe) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(push_chunk >= 256 && push_chunk <= (1L << 24), PIC_ERR_ARG, "push_chunk: 256 to 2^24 particles");
+    PIC_CHECK(xcd_stripe >= 1 && xcd_stripe <= 4096, PIC_ERR_ARG, "xcd_stripe: 1 to 4096 entries");
+    PIC_HIP(hipStreamSynchronize(s->w->stream));
+    s->push_chunk = push_chunk;
+    s->xcd_stripe = xcd_stripe;
     return PIC_OK;
 }
 

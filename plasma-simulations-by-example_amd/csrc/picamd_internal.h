@@ -41,7 +41,8 @@ constexpr int WNODES = WN * WN * WN;      // 2197
 constexpr int PUSH_THREADS = 256;
 constexpr int NDEST = 27;                 // destination tiles reachable in one step (3x3x3)
 constexpr int STAGE_WGS = 64;             // workgroups that walk the unbinned list (sort, diag)
-// The push gives each tile one workgroup per PUSH_CHUNK of its particles: flows that pile
+// The push gives each tile one workgroup per PUSH_CHUNK of its particles (the default; the
+// tuning table may set another per grid and particle count): flows that pile
 // ions into a few tiles (the focused wake behind the sphere reaches 40x the mean tile
 // population within 100 steps) would otherwise leave one workgroup running long after the
 // rest.  PUSH_STAGE_WGS workgroups push the unbinned list.
@@ -53,7 +54,8 @@ constexpr int PUSH_STAGE_WGS = 256;
 // atomics of particles that share a tile.
 constexpr int PLAN_STAGED_WGS = 1024;
 constexpr int PLAN_STAGED_SHARDS = 16;
-// Push workgroups are handed to the 8 XCDs in stripes of PUSH_XCD_STRIPE consecutive
+// Push workgroups are handed to the 8 XCDs in stripes of PUSH_XCD_STRIPE (default; tunable)
+// consecutive
 // (tile, chunk) entries, round-robin: each XCD's L2 serves runs of neighbouring tiles, and
 // every XCD gets a share of each region of the domain.  (Contiguous eighths per XCD left
 // the XCDs that own the wake running at twice the others' time.)
@@ -257,6 +259,10 @@ struct pic_species {
     int den_state = 0;            // 0 stale, 1 windows hold the last push deposit, 2 den final
     bool sorted = false;          // particles are all in bins (staging empty)
     long stage_added = 0;         // particles appended to staging since the last push or sort
+    // launch shapes of the push (from the tuning table when it has an entry for this size):
+    // particles per push workgroup, map entries per XCD stripe
+    long push_chunk = pic::PUSH_CHUNK;
+    int xcd_stripe = pic::PUSH_XCD_STRIPE;
 };
 
 namespace pic {
@@ -282,6 +288,10 @@ struct pic_solver {
     // Newton stopping test of the reduced solvers: RMS(y) < nr_tol (the reference's NR_TOL,
     // ch3/ver2/PotentialSolver.cpp:229; tightened by the differentiable run)
     double nr_tol = 1e-3;
+    // multigrid launch shapes set through pic_solver_set_mg (0: the tuning table's, else the
+    // built-in defaults)
+    int mg_sweeps = 0, mg_gal_from = 0;
+    long mg_fuse_max = 0;
 };
 
 // ---- error handling ------------------------------------------------------------------
@@ -289,6 +299,10 @@ namespace pic {
 void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 int grow_partials(pic_world *w, int nblocks);
+
+// the tuning table's value of `key` for the entry of `kind` nearest to (nodes, particles);
+// dflt without one (tuning.hip)
+double tuned(const char *kind, double nodes, double particles, const char *key, double dflt);
 
 // phase timing helpers (no-ops unless timing is enabled)
 void timer_begin(pic_world *w, int phase, hipEvent_t *a);

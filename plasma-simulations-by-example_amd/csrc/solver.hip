@@ -1447,6 +1447,21 @@ int pic_solver_solve(pic_solver *s, double *residual, int *iters) {
     return rc;
 }
 
+int pic_solver_set_mg(pic_solver *s, int sweeps, int gal_from, long fuse_max) {
+    PIC_CHECK(s, PIC_ERR_ARG, "null argument");
+    PIC_CHECK(sweeps >= 0 && sweeps <= 8 && gal_from >= 0 && gal_from <= 16 && fuse_max >= 0, PIC_ERR_ARG,
+              "multigrid shape out of range");
+    PIC_HIP(hipStreamSynchronize(s->w->stream));
+    s->mg_sweeps = sweeps;
+    s->mg_gal_from = gal_from;
+    s->mg_fuse_max = fuse_max;
+    if (s->mg) {  // rebuilt with the new shape by the next solve
+        mg_destroy(s->mg);
+        s->mg = nullptr;
+    }
+    return PIC_OK;
+}
+
 int pic_solver_get_info(pic_solver *s, int info[4]) {
     PIC_CHECK(s && info, PIC_ERR_ARG, "null argument");
     info[0] = s->info_newton;

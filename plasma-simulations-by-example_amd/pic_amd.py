@@ -46,6 +46,7 @@ EXPORTS = [
     "pic_ensemble_set_tuning", "pic_ensemble_solve_initial", "pic_ensemble_step",
     "pic_ensemble_get_field", "pic_ensemble_solver_status", "pic_ensemble_diag",
     "pic_ensemble_stats",
+    "pic_tuning_load", "pic_tuning_lookup", "pic_species_set_launch", "pic_solver_set_mg",
 ]
 
 
@@ -169,6 +170,11 @@ def lib():
             "pic_ensemble_solver_status": (C.c_int, [_vp, C.POINTER(C.c_int), _dp]),
             "pic_ensemble_diag": (C.c_int, [_vp, _dp]),
             "pic_ensemble_stats": (C.c_int, [_vp, _dp]),
+            "pic_tuning_load": (C.c_int, [C.c_char_p]),
+            "pic_tuning_lookup": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_char_p,
+                                            C.c_double, _dp]),
+            "pic_species_set_launch": (C.c_int, [_vp, C.c_long, C.c_int]),
+            "pic_solver_set_mg": (C.c_int, [_vp, C.c_int, C.c_int, C.c_long]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -373,6 +379,10 @@ class Species:
     def sort(self):
         _check(lib().pic_species_sort(self.h))
 
+    def set_launch(self, push_chunk, xcd_stripe):
+        """Push launch shape: particles per workgroup, map entries per XCD stripe."""
+        _check(lib().pic_species_set_launch(self.h, int(push_chunk), int(xcd_stripe)))
+
     def inject_beam(self, v_drift, den, T, dt, seed, counter):
         n = C.c_long(0)
         _check(lib().pic_species_inject_beam(self.h, v_drift, den, T, dt, seed, counter,
@@ -419,6 +429,10 @@ class PotentialSolver:
         a = (C.c_int * 4)()
         _check(lib().pic_solver_get_info(self.h, a))
         return {"newton": a[0], "iters": a[1], "fallbacks": a[2], "converged": bool(a[3])}
+
+    def set_mg(self, sweeps=0, gal_from=0, fuse_max=0):
+        """Multigrid shape (0: the tuning table's, else the default); rebuilt by the next solve."""
+        _check(lib().pic_solver_set_mg(self.h, int(sweeps), int(gal_from), int(fuse_max)))
 
     def solve_qn(self):
         _check(lib().pic_solver_solve_qn(self.h))
@@ -661,6 +675,25 @@ class Ensemble:
                 "nodes": int(out[3]), "push_workgroups": int(out[4])}
 
 
+def tuning_load(path=None):
+    """(Re)loads the tuning table (None: PICAMD_TUNING, else tuning/table.txt beside the
+    library)."""
+    _check(lib().pic_tuning_load(path.encode() if path else None))
+
+
+def tuning_lookup(kind, nodes, particles, key, default):
+    """The tuning table's value of key for kind nearest to (nodes, particles)."""
+    v = C.c_double(0)
+    _check(lib().pic_tuning_lookup(kind.encode(), float(nodes), float(particles), key.encode(),
+                                   float(default), C.byref(v)))
+    return v.value
+
+
 def tuned_ensemble_params(ens):
-    """Launch shapes for an ensemble: the defaults until a tuning table is present."""
-    return dict(chunk_particles=4096, window_nodes=6144, sor_threads=1024, sort_every=20)
+    """The launch shapes the library took from the tuning table for this ensemble."""
+    st = ens.stats()
+    n, p = st["nodes"], max(st["particles"], 1)
+    return dict(chunk_particles=int(tuning_lookup("ensemble", n, p, "chunk", 4096)),
+                window_nodes=int(tuning_lookup("ensemble", n, p, "window", 6144)),
+                sor_threads=int(tuning_lookup("ensemble", n, p, "sor_threads", 1024)),
+                sort_every=int(tuning_lookup("ensemble", n, p, "sort_every", 20)))
