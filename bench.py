@@ -103,6 +103,7 @@ def push_only(P, args, grid):
     prec = P.MIXED if args.precision == "mixed" else P.FP64
     ions = P.Species(w, 16 * AMU, QE, Case.n0 * vol / args.particles, prec,
                      capacity=int(args.particles * 1.02))
+    ions.set_frame_velocity((0.0, 0.0, Case.v_drift))  # as the beam source sets it
     ions.load_box_device(Case.x0, Case.xm, Case.n0, int(args.particles), Case.v_drift, args.T_ion,
                          seed=1)
     s = P.PotentialSolver(w, P.QN, 1, 1e-4)
@@ -177,6 +178,7 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     vol = np.prod(np.subtract(Case.xm, Case.x0))
     mpw0 = Case.n0 * vol / nparticles
     ions = P.Species(w, 16 * AMU, QE, mpw0, precision, capacity=int(nparticles * 1.02), name="O+")
+    ions.set_frame_velocity((0.0, 0.0, Case.v_drift))  # as the beam source sets it
     t0 = time.time()
     ions.load_box_device(Case.x0, Case.xm, Case.n0, int(nparticles), Case.v_drift, T_ion, seed=1)
     w.sync()

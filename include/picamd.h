@@ -30,8 +30,9 @@ extern "C" {
 #endif
 
 /* 2: PIC_T_COUNT grew to 8 (PIC_T_PUSH_KERNEL); 3: pic_solver_get_info; 4: pic_adjoint_*;
- * 5: pic_ensemble_*; 6: tuning table, pic_species_set_launch, pic_solver_set_mg */
-#define PICAMD_ABI_VERSION 6
+ * 5: pic_ensemble_*; 6: tuning table, pic_species_set_launch, pic_solver_set_mg;
+ * 7: pic_species_set_frame_velocity */
+#define PICAMD_ABI_VERSION 7
 
 /* status codes */
 #define PIC_OK 0
@@ -178,6 +179,13 @@ int pic_species_sort(pic_species *s);
  * particles per push workgroup and push-map entries per XCD stripe.  Species take both from
  * the tuning table at creation (else 32768 and 32). */
 int pic_species_set_launch(pic_species *s, long push_chunk, int xcd_stripe);
+/* No counterpart in the reference (a storage choice of the mixed form): a PIC_MIXED species
+ * stores each velocity as v - u in FP32, u its frame velocity (default 0).  A drifting
+ * species (a beam at v_drift along z) keeps FP32 rounding at the scale of its deviations
+ * from the drift instead of the drift itself.  Every interface still takes and returns v;
+ * stored particles are re-expressed in the new frame.  PIC_FP64 species store v and ignore
+ * the call.  The C++ Source classes set u = v_drift z on a species they feed. */
+int pic_species_set_frame_velocity(pic_species *s, const double u[3]);
 /* ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27) when T == 0, WarmBeamSource::sample
  * (ch4/Source.cpp:31-56) when T > 0 [K], on the device: num = (int)(den*v*A*dt/mpw0 + U)
  * particles uniform on the k=0 face, v = v_drift z (+ isotropic thermal speed), then the

@@ -99,6 +99,7 @@ def tune_push(args):
     n = int(args.particles)
     vol = np.prod(np.subtract(Case.xm, Case.x0))
     ions = P.Species(w, 16 * AMU, QE, Case.n0 * vol / n, P.MIXED, capacity=int(n * 1.02))
+    ions.set_frame_velocity((0.0, 0.0, Case.v_drift))
 
     def load():
         # replace every particle (a device load appends): empty the species, then load
@@ -139,6 +140,7 @@ def tune_mg(args):
     n = int(args.particles)
     vol = np.prod(np.subtract(Case.xm, Case.x0))
     ions = P.Species(w, 16 * AMU, QE, Case.n0 * vol / n, P.MIXED, capacity=int(n * 1.02))
+    ions.set_frame_velocity((0.0, 0.0, Case.v_drift))
     ions.load_box_device(Case.x0, Case.xm, Case.n0, n, Case.v_drift, 1000.0, seed=1)
     ions.compute_number_density()
     w.compute_charge_density([ions])

@@ -33,6 +33,10 @@ struct PushParams {
     // PIC_MIXED (FP32, cell-relative units)
     float qm_dt;     // dt*q/m
     float dt_dh[3];  // dt/dh
+    // frame velocity u: the record holds v - u; a step moves the offset by
+    // fdh_hi + ((v - u) dt/dh + fdh_lo), u dt/dh split over two floats; a wall reverses
+    // v, so the stored v - u becomes -(v - u) - 2u
+    float fdh_hi[3], fdh_lo[3], fr2[3];
     int sph_c[3];    // sphere centre in cells: integer part
     float sph_f[3];  //                          fractional part
     float dhf[3];
@@ -216,7 +220,7 @@ __device__ inline void advance_axis(int &c, float &o, float d) {
 // L < 0 -> -L, L >= nc -> 2nc - L, velocity reversed.  A particle that lands exactly on the
 // upper wall (L == nc after reflection, where the reference scatters past the array) is
 // clamped just inside it.
-__device__ inline void reflect_axis(int &c, float &o, float &v, int nc) {
+__device__ inline void reflect_axis(int &c, float &o, float &v, int nc, float u2) {
     if (c < 0) {
         c = -c - 1;
         o = 1.0f - o;
@@ -224,7 +228,7 @@ __device__ inline void reflect_axis(int &c, float &o, float &v, int nc) {
             o = 0.0f;
             c += 1;
         }
-        v = -v;
+        v = -v - u2;
     } else if (c >= nc) {
         c = 2 * nc - c - 1;
         o = 1.0f - o;
@@ -232,7 +236,7 @@ __device__ inline void reflect_axis(int &c, float &o, float &v, int nc) {
             o = 0.0f;
             c += 1;
         }
-        v = -v;
+        v = -v - u2;
     }
     if (c >= nc) {
         c = nc - 1;
@@ -287,13 +291,13 @@ __device__ inline bool push_mixed(const PushParams &P, PMixed &p, float3 E) {
     p.vx += E.x * P.qm_dt;
     p.vy += E.y * P.qm_dt;
     p.vz += E.z * P.qm_dt;
-    advance_axis(p.ci, p.ox, p.vx * P.dt_dh[0]);
-    advance_axis(p.cj, p.oy, p.vy * P.dt_dh[1]);
-    advance_axis(p.ck, p.oz, p.vz * P.dt_dh[2]);
+    advance_axis(p.ci, p.ox, P.fdh_hi[0] + (p.vx * P.dt_dh[0] + P.fdh_lo[0]));
+    advance_axis(p.cj, p.oy, P.fdh_hi[1] + (p.vy * P.dt_dh[1] + P.fdh_lo[1]));
+    advance_axis(p.ck, p.oz, P.fdh_hi[2] + (p.vz * P.dt_dh[2] + P.fdh_lo[2]));
     if (P.bc == PIC_BC_REFLECT) {
-        reflect_axis(p.ci, p.ox, p.vx, P.g.nci);
-        reflect_axis(p.cj, p.oy, p.vy, P.g.ncj);
-        reflect_axis(p.ck, p.oz, p.vz, P.g.nck);
+        reflect_axis(p.ci, p.ox, p.vx, P.g.nci, P.fr2[0]);
+        reflect_axis(p.cj, p.oy, p.vy, P.g.ncj, P.fr2[1]);
+        reflect_axis(p.ck, p.oz, p.vz, P.g.nck, P.fr2[2]);
         return true;
     }
     // absorbing: !inBounds (half-open [x0,xm)) || inSphere  (ch3/ver2/Species.cpp:28)
@@ -1276,7 +1280,7 @@ template <int PREC>
 __global__ void __launch_bounds__(256)
     k_append(Grid g, long n, const double *pos, const double *vel, const double *mpw, const uint32_t *ids,
              uint32_t id0, int rewind, double qm, double half_dt, const float4 *ef32,
-             const double4 *ef64, PartView dst, uint32_t stage0) {
+             const double4 *ef64, PartView dst, uint32_t stage0, double3 frame) {
 #pragma clang fp contract(off)
     long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -1322,9 +1326,9 @@ __global__ void __launch_bounds__(256)
         p.ox = fo[0];
         p.oy = fo[1];
         p.oz = fo[2];
-        p.vx = (float)vx;
-        p.vy = (float)vy;
-        p.vz = (float)vz;
+        p.vx = (float)(vx - frame.x);
+        p.vy = (float)(vy - frame.y);
+        p.vz = (float)(vz - frame.z);
         p.w = (float)mpw[q];
         p.id = ids ? ids[q] : id0 + (uint32_t)q;
         store_p(dst, idx, p);
@@ -1338,7 +1342,7 @@ __global__ void __launch_bounds__(256)
 template <int PREC>
 __global__ void __launch_bounds__(256)
     k_gather_out(Grid g, PartView in, const uint32_t *offsets, double *pos, double *vel, double *mpw,
-                 uint32_t *ids) {
+                 uint32_t *ids, double3 frame) {
     using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
     Share s = share_of(g, in);
     uint32_t o = offsets[blockIdx.x < (unsigned)g.ntiles ? blockIdx.x : g.ntiles];
@@ -1355,11 +1359,31 @@ __global__ void __launch_bounds__(256)
             pos[3 * d + 1] = p.y;
             pos[3 * d + 2] = p.z;
         }
-        vel[3 * d + 0] = p.vx;
-        vel[3 * d + 1] = p.vy;
-        vel[3 * d + 2] = p.vz;
+        if constexpr (PREC == PIC_MIXED) {
+            vel[3 * d + 0] = frame.x + (double)p.vx;
+            vel[3 * d + 1] = frame.y + (double)p.vy;
+            vel[3 * d + 2] = frame.z + (double)p.vz;
+        } else {
+            vel[3 * d + 0] = p.vx;
+            vel[3 * d + 1] = p.vy;
+            vel[3 * d + 2] = p.vz;
+        }
         mpw[d] = p.w;
         if (ids) ids[d] = p.id;
+    }
+}
+
+// PIC_MIXED records re-expressed in another frame: v - u_old -> v - u_new, du = u_old - u_new.
+__global__ void __launch_bounds__(256) k_shift_frame(Grid g, PartView in, double3 du) {
+    Share s = share_of(g, in);
+    for (uint32_t q = s.first; q < s.cnt; q += s.step) {
+        PMixed p;
+        long idx = share_index(s, q);
+        load_p(in, idx, p);
+        p.vx = (float)((double)p.vx + du.x);
+        p.vy = (float)((double)p.vy + du.y);
+        p.vz = (float)((double)p.vz + du.z);
+        store_p(in, idx, p);
     }
 }
 
@@ -1378,6 +1402,7 @@ struct InjectParams {
     int rewind;
     uint32_t key0, key1, ctr_lo, ctr_hi;
     uint32_t id0;
+    double frame[3];  // PIC_MIXED records hold v - frame
 };
 
 template <int PREC>
@@ -1464,9 +1489,9 @@ __global__ void __launch_bounds__(256)
         p.ox = fo[0];
         p.oy = fo[1];
         p.oz = fo[2];
-        p.vx = (float)vx;
-        p.vy = (float)vy;
-        p.vz = (float)vz;
+        p.vx = (float)(vx - I.frame[0]);
+        p.vy = (float)(vy - I.frame[1]);
+        p.vz = (float)(vz - I.frame[2]);
         p.w = (float)I.w;
         p.id = I.id0 + (uint32_t)q;
         store_p(dst, idx, p);
@@ -1935,10 +1960,12 @@ static int append_host(pic_species *s, long n, const double *pos, const double *
     double qm = s->charge / s->mass;
     if (s->prec == PIC_MIXED)
         k_append<PIC_MIXED><<<nb256(n), 256, 0, w->stream>>>(w->g, n, dpos, dvel, dmpw, dids, s->next_id, rewind,
-                                                             qm, 0.5 * dt, w->ef32, w->ef64, view_of(b), stage);
+                                                             qm, 0.5 * dt, w->ef32, w->ef64, view_of(b), stage,
+                                                             frame_of(s));
     else
         k_append<PIC_FP64><<<nb256(n), 256, 0, w->stream>>>(w->g, n, dpos, dvel, dmpw, dids, s->next_id, rewind,
-                                                            qm, 0.5 * dt, w->ef32, w->ef64, view_of(b), stage);
+                                                            qm, 0.5 * dt, w->ef32, w->ef64, view_of(b), stage,
+                                                            frame_of(s));
     PIC_LAUNCH_CHECK();
     k_set_u32<<<1, 1, 0, w->stream>>>(b.stage_count, stage + (uint32_t)n);
     PIC_HIP(hipStreamSynchronize(w->stream));
@@ -2063,9 +2090,9 @@ int pic_species_get_particles(pic_species *s, double *pos, double *vel, double *
     unsigned grid = g.ntiles + STAGE_WGS;
     PartView v = view_of(s->buf[s->cur]);
     if (s->prec == PIC_MIXED)
-        k_gather_out<PIC_MIXED><<<grid, 256, 0, w->stream>>>(g, v, doff, dpos, dvel, dmpw, dids);
+        k_gather_out<PIC_MIXED><<<grid, 256, 0, w->stream>>>(g, v, doff, dpos, dvel, dmpw, dids, frame_of(s));
     else
-        k_gather_out<PIC_FP64><<<grid, 256, 0, w->stream>>>(g, v, doff, dpos, dvel, dmpw, dids);
+        k_gather_out<PIC_FP64><<<grid, 256, 0, w->stream>>>(g, v, doff, dpos, dvel, dmpw, dids, frame_of(s));
     PIC_LAUNCH_CHECK();
     PIC_HIP(hipMemcpyAsync(pos, dpos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, w->stream));
     PIC_HIP(hipMemcpyAsync(vel, dvel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, w->stream));
@@ -2140,6 +2167,12 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     }
     P.r2f = (float)w->sphere.r2;
     P.sph_r2 = w->sphere.r2;
+    for (int c = 0; c < 3; c++) {
+        double u = s->frame[c] * dt / g.dh[c];
+        P.fdh_hi[c] = (float)u;
+        P.fdh_lo[c] = (float)(u - (double)P.fdh_hi[c]);
+        P.fr2[c] = (float)(2 * s->frame[c]);
+    }
     PushOut O{};
     O.chunk_map = s->chunk_map;
     O.total_groups = plan_tmp(s).total;
@@ -2247,6 +2280,25 @@ int pic_species_update_averages(pic_species *s) {
     return PIC_OK;
 }
 
+// getRealCount, getMomentum = mass*sum(v*mpw), getKE = 0.5*mass*sum(mpw*v^2) from the
+// reduced sums W = sum(mpw), S = sum(mpw*v), Q = sum(mpw*|v|^2) of the stored velocities.
+// PIC_MIXED stores v - u (u the species frame): sum(mpw*v) = S + u W and
+// sum(mpw*|v|^2) = Q + 2 u.S + |u|^2 W.
+static void finish_diag(const pic_species *s, const double *h, double out[5]) {
+    const double *u = s->frame;
+    bool moving = s->prec == PIC_MIXED && (u[0] != 0 || u[1] != 0 || u[2] != 0);
+    double S[3] = {h[1], h[2], h[3]}, Q = h[4];
+    if (moving) {
+        Q += 2 * (u[0] * S[0] + u[1] * S[1] + u[2] * S[2]) + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * h[0];
+        for (int c = 0; c < 3; c++) S[c] += u[c] * h[0];
+    }
+    out[0] = h[0];
+    out[1] = S[0] * s->mass;
+    out[2] = S[1] * s->mass;
+    out[3] = S[2] * s->mass;
+    out[4] = 0.5 * s->mass * Q;
+}
+
 int pic_species_diag(pic_species *s, double out[5]) {
     PIC_CHECK(s && out, PIC_ERR_ARG, "null argument");
     pic_world *w = s->w;
@@ -2265,12 +2317,24 @@ int pic_species_diag(pic_species *s, double out[5]) {
     PIC_LAUNCH_CHECK();
     PIC_HIP(hipMemcpyAsync(s->diag_host, s->diag_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, w->stream));
     PIC_HIP(hipStreamSynchronize(w->stream));
-    // getRealCount, getMomentum = mass*sum(v*mpw), getKE = 0.5*mass*sum(mpw*v^2)
-    out[0] = s->diag_host[0];
-    out[1] = s->diag_host[1] * s->mass;
-    out[2] = s->diag_host[2] * s->mass;
-    out[3] = s->diag_host[3] * s->mass;
-    out[4] = 0.5 * s->mass * s->diag_host[4];
+    finish_diag(s, s->diag_host, out);
+    return PIC_OK;
+}
+
+int pic_species_set_frame_velocity(pic_species *s, const double u[3]) {
+    PIC_CHECK(s && u, PIC_ERR_ARG, "null argument");
+    for (int c = 0; c < 3; c++) PIC_CHECK(std::isfinite(u[c]), PIC_ERR_ARG, "frame velocity not finite");
+    if (s->prec != PIC_MIXED) return PIC_OK;  // FP64 records hold v itself
+    double du[3] = {s->frame[0] - u[0], s->frame[1] - u[1], s->frame[2] - u[2]};
+    if (du[0] == 0 && du[1] == 0 && du[2] == 0) return PIC_OK;
+    pic_world *w = s->w;
+    if (s->np_upper > 0) {
+        k_shift_frame<<<(unsigned)(w->g.ntiles + STAGE_WGS), 256, 0, w->stream>>>(
+            w->g, view_of(s->buf[s->cur]), make_double3(du[0], du[1], du[2]));
+        PIC_LAUNCH_CHECK();
+    }
+    for (int c = 0; c < 3; c++) s->frame[c] = u[c];
+    s->diag_valid = false;  // partial sums of the old frame's velocities
     return PIC_OK;
 }
 
@@ -2314,17 +2378,14 @@ int pic_species_diag_noalloc(pic_species *s, double out[5]) {
     double h[NDIAG_SLOTS];
     PIC_HIP(hipMemcpyAsync(h, ptr + DIAG_NOALLOC_SUMS, sizeof h, hipMemcpyDeviceToHost, w->stream));
     PIC_HIP(hipStreamSynchronize(w->stream));
-    out[0] = h[0];
-    out[1] = h[1] * s->mass;
-    out[2] = h[2] * s->mass;
-    out[3] = h[3] * s->mass;
-    out[4] = 0.5 * s->mass * h[4];
+    finish_diag(s, h, out);
     return PIC_OK;
 }
 
 static int inject_common(pic_species *s, InjectParams &I, bool flat_ok) {
     (void)flat_ok;
     pic_world *w = s->w;
+    for (int c = 0; c < 3; c++) I.frame[c] = s->frame[c];
     if (I.n <= 0) return PIC_OK;
     uint32_t stage = 0;
     // the staging count comes from the last push's snapshot when it has landed (the solve

@@ -263,7 +263,11 @@ struct pic_species {
     // particles per push workgroup, map entries per XCD stripe
     long push_chunk = pic::PUSH_CHUNK;
     int xcd_stripe = pic::PUSH_XCD_STRIPE;
+    // PIC_MIXED: the records hold v - frame in FP32 (FP64 species store v itself)
+    double frame[3] = {0, 0, 0};
 };
+
+inline double3 frame_of(const pic_species *s) { return make_double3(s->frame[0], s->frame[1], s->frame[2]); }
 
 namespace pic {
 struct MgHierarchy;

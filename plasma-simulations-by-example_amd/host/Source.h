@@ -1,6 +1,8 @@
 // Injection sources on the device: ColdBeamSource (ch3/ver2/Source.h) and WarmBeamSource
 // (ch4/Source.h).  Each sample() draws its random numbers from a counter-based generator
-// keyed by a seed taken from the global rnd at construction, so a seeded run repeats.
+// keyed by a seed taken from the global rnd at construction, so a seeded run repeats.  A
+// source sets its species' frame velocity to its drift, so that a mixed-precision species
+// rounds its FP32 velocities at the scale of the deviations from the beam.
 #ifndef PICAMD_HOST_SOURCE_H
 #define PICAMD_HOST_SOURCE_H
 
@@ -15,8 +17,7 @@ public:
 
 class ColdBeamSource : public Source {
 public:
-    ColdBeamSource(Species &species, World &world, double v_drift, double den)
-        : sp{&species}, world{&world}, v_drift{v_drift}, den{den}, seed{world.sourceSeed()} {}
+    ColdBeamSource(Species &species, World &world, double v_drift, double den);
     void sample() override;
 
 protected:
@@ -30,8 +31,7 @@ protected:
 
 class WarmBeamSource : public Source {
 public:
-    WarmBeamSource(Species &species, World &world, double v_drift, double den, double T)
-        : sp{&species}, world{&world}, v_drift{v_drift}, den{den}, T{T}, seed{world.sourceSeed()} {}
+    WarmBeamSource(Species &species, World &world, double v_drift, double den, double T);
     void sample() override;
 
 protected:

@@ -47,6 +47,7 @@ EXPORTS = [
     "pic_ensemble_get_field", "pic_ensemble_solver_status", "pic_ensemble_diag",
     "pic_ensemble_stats",
     "pic_tuning_load", "pic_tuning_lookup", "pic_species_set_launch", "pic_solver_set_mg",
+    "pic_species_set_frame_velocity",
 ]
 
 
@@ -174,6 +175,7 @@ def lib():
             "pic_tuning_lookup": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_char_p,
                                             C.c_double, _dp]),
             "pic_species_set_launch": (C.c_int, [_vp, C.c_long, C.c_int]),
+            "pic_species_set_frame_velocity": (C.c_int, [_vp, _dp]),
             "pic_solver_set_mg": (C.c_int, [_vp, C.c_int, C.c_int, C.c_long]),
         }
         for name, (res, args) in sig.items():
@@ -382,6 +384,10 @@ class Species:
     def set_launch(self, push_chunk, xcd_stripe):
         """Push launch shape: particles per workgroup, map entries per XCD stripe."""
         _check(lib().pic_species_set_launch(self.h, int(push_chunk), int(xcd_stripe)))
+
+    def set_frame_velocity(self, u):
+        """PIC_MIXED velocities are stored as v - u in FP32 (a beam: u = its drift)."""
+        _check(lib().pic_species_set_frame_velocity(self.h, _d(_v3(u))))
 
     def inject_beam(self, v_drift, den, T, dt, seed, counter):
         n = C.c_long(0)

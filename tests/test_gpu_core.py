@@ -102,7 +102,8 @@ def test_solve_qn_matches_closed_form():
 
 
 def _push_case(prec, bc, steps, dt, vmax, sphere=None, n=20000, grid=(21, 21, 21), x0=C1_X0,
-               xm=C1_XM, seed=11, amp=200.0, charge=-O.QE, mass=O.ME, box=None):
+               xm=C1_XM, seed=11, amp=200.0, charge=-O.QE, mass=O.ME, box=None, drift=None,
+               frame=None, frame_after=False):
     w, g = make_world(grid, x0, xm)
     phi, ef = smooth_field(g, seed, amp)
     if sphere is not None:
@@ -110,11 +111,17 @@ def _push_case(prec, bc, steps, dt, vmax, sphere=None, n=20000, grid=(21, 21, 21
     w.set_field(P.EF, ef)
     sp = P.Species(w, mass, charge, 1e4, prec, capacity=n, track_ids=True)
     pos, vel, mpw = random_particles(g, n, seed + 1, vmax, box)
+    if drift is not None:
+        vel = vel + np.asarray(drift)
     if sphere is not None:  # start outside the sphere
         keep = np.sum((pos - np.asarray(sphere[0])) ** 2, axis=1) > sphere[1] ** 2 * 1.02
         pos, vel, mpw = pos[keep], vel[keep], mpw[keep]
     ids = np.arange(len(mpw), dtype=np.uint32)
+    if frame is not None and not frame_after:
+        sp.set_frame_velocity(frame)
     sp.set_particles(pos, vel, mpw, ids)
+    if frame is not None and frame_after:
+        sp.set_frame_velocity(frame)
     # the device stores what it was given: read back the starting state (mixed rounds it)
     p0, v0, w0, i0 = sp.particles()
     order = np.argsort(i0)
@@ -218,6 +225,46 @@ def test_push_absorb_sphere_mixed_matches_oracle():
     assert dcell.max() < 2e-4, dcell.max()
     assert np.sqrt(np.mean(dcell ** 2)) < 1e-5
     assert rel_l2(r["den"], r["den_ref"]) < 1e-4
+
+
+@pytest.mark.parametrize("bc", [P.REFLECT, P.ABSORB])
+def test_push_mixed_frame_velocity_matches_oracle(bc):
+    """A drifting mixed species with its frame velocity at the drift
+    (pic_species_set_frame_velocity): every interface still sees v, the push still matches
+    the oracle, and the FP32 velocity error falls from the scale of the drift (3e6 m/s) to
+    that of the deviations from it (2e5 m/s; a weak field keeps them there over the steps)."""
+    drift = (0.0, 0.0, 3e6)
+    sphere = ((0.0, 0.0, 0.0), 0.05, -100.0) if bc == P.ABSORB else None
+    kw = dict(sphere=sphere, drift=drift, amp=0.5)
+    plain = _push_case(P.MIXED, bc, 10, 1e-9, 2e5, **kw)
+    framed = _push_case(P.MIXED, bc, 10, 1e-9, 2e5, frame=drift, **kw)
+    errs = []
+    for r in (plain, framed):
+        common = np.intersect1d(r["gi"], r["ri"])
+        assert len(r["ri"]) - len(common) <= 2 and len(r["gi"]) - len(common) <= 2
+        gm, rm = np.isin(r["gi"], common), np.isin(r["ri"], common)
+        dcell = np.abs(r["gp"][gm] - r["rp"][rm]) / r["g"].dh
+        assert dcell.max() < 2e-4, dcell.max()
+        assert rel_l2(r["den"], r["den_ref"]) < 1e-4
+        d = O.species_diag(r["rv"], r["rw"], O.ME)
+        np.testing.assert_allclose(r["diag"][[0, 3, 4]], d[[0, 3, 4]], rtol=1e-6)
+        # the drift's component: x and y hold only the deviations in either form
+        errs.append(float(np.median(np.abs(r["gv"][gm, 2] - r["rv"][rm, 2]))))
+    assert errs[1] < errs[0] / 4, errs
+
+
+def test_frame_velocity_change_keeps_particles():
+    """Setting the frame on a species that holds particles re-expresses them: read-back
+    velocities and diagnostics unchanged to FP32 rounding of the stored value."""
+    drift = (1e5, -2e5, 3e6)
+    a = _push_case(P.MIXED, P.REFLECT, 0, 1e-9, 2e5, drift=drift, frame=drift)
+    b = _push_case(P.MIXED, P.REFLECT, 0, 1e-9, 2e5, drift=drift, frame=drift, frame_after=True)
+    assert np.array_equal(a["gi"], b["gi"])
+    np.testing.assert_allclose(b["gv"], a["gv"], rtol=0, atol=0.5)  # ulp(3e6) in FP32: 0.25
+    np.testing.assert_allclose(b["diag"], a["diag"], rtol=1e-6)
+    b["sp"].set_frame_velocity((0.0, 0.0, 0.0))
+    v0 = b["sp"].particles()[1][np.argsort(b["sp"].particles()[3])]
+    np.testing.assert_allclose(v0, b["gv"], rtol=0, atol=0.5)
 
 
 # The wide-rim push (a three-cell window rim for the tiles whose particles left the two-cell
