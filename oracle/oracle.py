@@ -138,6 +138,7 @@ def lib():
                                            C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_int, _dp]),
             "orc_dsph_run": (C.c_int, [C.POINTER(DsphConfig), _dp, _dp, _dp, _dp, _dp]),
+            "orc_update_average": (None, [C.c_long, _dp, _dp, _ip]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -301,6 +302,23 @@ def cold_beam_sample(g: OGrid, P: Particles, rng: Rng, v_drift, den, mpw0, ef3, 
     return r
 
 
+def cold_beam_draws(g: OGrid, rng: Rng, v_drift, den, mpw0, dt):
+    """The random draws of ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27) without its
+    addParticle calls: the count (int)(n v A dt / mpw0 + rnd()), then x and y of each
+    particle on the z = x0 face.  Returns the (n, 3) positions; the caller adds them with
+    velocity (0, 0, v_drift) and weight mpw0."""
+    Lx = g.dh[0] * (g.ni - 1)
+    Ly = g.dh[1] * (g.nj - 1)
+    num_real = den * v_drift * (Lx * Ly) * dt
+    n = int(num_real / mpw0 + rng.uniform())
+    pos = np.empty((n, 3))
+    for i in range(n):
+        pos[i, 0] = g.x0[0] + rng.uniform() * Lx
+        pos[i, 1] = g.x0[1] + rng.uniform() * Ly
+        pos[i, 2] = g.x0[2]
+    return pos
+
+
 def warm_beam_sample(g: OGrid, P: Particles, rng: Rng, v_drift, den, T, mpw0, ef3, dt, charge,
                      mass):
     r = lib().orc_warm_beam_sample(g.ref, C.byref(rng.s), v_drift, den, T, mpw0, _d(ef3), dt,
@@ -368,6 +386,43 @@ def species_diag(vel, mpw, mass):
     mpw = np.ascontiguousarray(mpw, dtype=np.float64)
     lib().orc_species_diag(len(mpw), _d(vel), _d(mpw), mass, _d(out))
     return out
+
+
+def update_average(ave, inst, samples):
+    """Field::updateAverage (ch3/ver2/Field.h:214-220) in place; returns the new sample count."""
+    n = C.c_int(samples)
+    lib().orc_update_average(len(ave), _d(ave), _d(np.ascontiguousarray(inst, dtype=np.float64)),
+                             C.byref(n))
+    return n.value
+
+
+class SteadyState:
+    """World::steadyState (ch3/ver2/World.cpp:129-158).  Each call forms the totals of the
+    species' real counts, of the magnitudes of their z momenta and of PE plus their kinetic
+    energies, in that order of summation; the state turns steady when all three differ from
+    the previous call's by less than 1e-3 relative, and stays steady (the totals are then no
+    longer formed).  species_diags holds one [real_count, px, py, pz, KE] row per species."""
+
+    TOL = 1e-3
+
+    def __init__(self):
+        self.steady = False
+        self.last = (0.0, 0.0, 0.0)
+
+    def __call__(self, species_diags, pe):
+        if self.steady:
+            return True
+        mass, mom, en = 0.0, 0.0, pe
+        for d in species_diags:
+            mass += d[0]
+            mom += abs(d[3])
+            en += d[4]
+        lm, lp, le = self.last
+        if abs((mass - lm) / mass) < self.TOL and abs((mom - lp) / mom) < self.TOL and \
+                abs((en - le) / en) < self.TOL:
+            self.steady = True
+        self.last = (mass, mom, en)
+        return self.steady
 
 
 def philox4x32_10(ctr, key):

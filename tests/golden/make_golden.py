@@ -14,6 +14,12 @@ Files
   qs_ch2.npz           Species::loadParticlesBoxQS on an 11^3 particle lattice
   sphere_ch3.npz       ch3/ver2 sphere (13x13x25), cold beam (seed 7), 12 steps for each
                        solver type GS / PCG / QN
+  hist_c1.npz          config 1 as c1_ch2 but SOR tol 1e-8, 2000 steps: per-step diagnostics
+                       (getRealCount, getMomentum, getKE of both species) and getPE
+  hist_ch3.npz         ch3/ver2/Main.cpp's sphere run (21x21x41, -100 V, cold beam 7 km/s,
+                       seed 7) with the QN solver, dt 2e-7, mpw0 200, 400 steps: per-step
+                       particle count, diagnostics, getPE and World::steadyState (averages
+                       from the step it holds); den_ave and phi at the end
 """
 import os
 import sys
@@ -132,6 +138,68 @@ def gen_sphere():
     np.savez_compressed(os.path.join(HERE, "sphere_ch3.npz"), **out)
 
 
+def gen_hist_c1(steps=2000, tol=1e-8):
+    sim = R.RefSim("ch2", *C1["n"], C1["x0"], C1["xm"], C1["dt"], steps)
+    i = sim.add_species("O+", 16 * AMU, QE)
+    e = sim.add_species("e-", ME, -QE)
+    sim.reseed(1)
+    sim.load_box(i, C1["x0"], C1["xm"], 1e11, 80000)
+    sim.load_box(e, (0.0, 0.0, 0.0), C1["xm"], 1e11, 10000)
+    sim.make_solver(0, 10000, tol)
+    sim.solve()
+    sim.compute_ef()
+    diag, pe = [], []
+    while sim.advance_time():
+        for s in (i, e):
+            sim.advance(s)
+            sim.compute_number_density(s)
+        sim.compute_charge_density()
+        sim.solve()
+        sim.compute_ef()
+        diag.append(np.stack([sim.species_diag(i), sim.species_diag(e)]))
+        pe.append(sim.pe())
+    np.savez_compressed(os.path.join(HERE, "hist_c1.npz"), diag=np.array(diag), pe=np.array(pe),
+                        tol=tol, steps=steps)
+    sim.close()
+
+
+def gen_hist_ch3(steps=400, dt=2e-7, mpw0=200.0):
+    # ch3/ver2/Main.cpp:18-41 (grid, extents, sphere, inlet, beam, QN) at twice its dt and four
+    # times its macro-particle weight, so that the flow crosses the domain and reaches
+    # World::steadyState within the 400 steps
+    n, x0, xm = (21, 21, 41), (-0.1, -0.1, 0.0), (0.1, 0.1, 0.4)
+    sim = R.RefSim("ch3", *n, x0, xm, dt, steps)
+    sim.add_sphere((0, 0, 0.15), 0.05, -100.0)
+    sim.add_inlet()
+    ion = sim.add_species("O+", 16 * AMU, QE, mpw0)
+    sim.add_cold_source(ion, 7000.0, 1e10)
+    sim.make_solver(R.QN, 20000, 1e-4)
+    sim.set_reference_values(0.0, 1.5, 1e10)
+    sim.reseed(7)
+    sim.solve()
+    sim.compute_ef()
+    nps, diags, pes, steady = [], [], [], []
+    while sim.advance_time():
+        sim.sample_sources()
+        sim.advance(ion)
+        sim.compute_number_density(ion)
+        sim.compute_charge_density()
+        sim.solve()
+        sim.compute_ef()
+        st = sim.steady_state()
+        if st:
+            sim.update_averages(ion)
+        nps.append(sim.np_(ion))
+        diags.append(sim.species_diag(ion))
+        pes.append(sim.pe())
+        steady.append(st)
+    np.savez_compressed(os.path.join(HERE, "hist_ch3.npz"), np=np.array(nps), diag=np.array(diags),
+                        pe=np.array(pes), steady=np.array(steady, dtype=np.int8),
+                        den_ave=sim.den_ave(ion), phi=sim.field(R.PHI), steps=steps, dt=dt,
+                        mpw0=mpw0)
+    sim.close()
+
+
 if __name__ == "__main__":
     if not (R.available("ch2") and R.available("ch3")):
         R.build()
@@ -139,6 +207,8 @@ if __name__ == "__main__":
     gen_qs()
     gen_c1()
     gen_sphere()
+    gen_hist_c1()
+    gen_hist_ch3()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -137,3 +137,111 @@ def test_sphere_twelve_steps_bit_exact(name):
     assert np.array_equal(rho, G[f"{name}_rho"])
     assert np.array_equal(P.pos[:P.n][::41], G[f"{name}_pos"])
     assert np.array_equal(P.vel[:P.n][::41], G[f"{name}_vel"])
+
+
+def sphere_history_world():
+    """ch3/ver2/Main.cpp:18-45: the 21x21x41 sphere world, the PotentialSolver constructor's
+    QN solve with its default n0, then setReferenceValues(0, 1.5, 1e10) and solve()."""
+    x0, xm = (-0.1, -0.1, 0.0), (0.1, 0.1, 0.4)
+    g = O.OGrid(21, 21, 41, x0, xm)
+    oid = np.zeros(g.nn, dtype=np.int32)
+    phi = g.zeros()
+    O.add_sphere(g, oid, phi, (0, 0, 0.15), 0.05, -100.0)
+    O.add_inlet(g, oid, phi)
+    O.solve_qn(g, phi, g.zeros(), oid, 0, 1.5, 1e12)
+    O.solve_qn(g, phi, g.zeros(), oid, 0, 1.5, 1e10)
+    return g, oid, phi
+
+
+def test_cold_beam_draws_match_sample():
+    """The host-side draws of ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27) followed by
+    addParticle give the oracle's cold_beam_sample, draw for draw, over several calls."""
+    g, oid, phi = sphere_history_world()
+    ef = O.compute_ef(g, phi)
+    dt, mpw0, m = 2e-7, 200.0, 16 * AMU
+    ra, rb = O.Rng(7), O.Rng(7)
+    Pa, Pb = O.Particles(20000), O.Particles(20000)
+    for _ in range(5):
+        O.cold_beam_sample(g, Pa, ra, 7000.0, 1e10, mpw0, ef, dt, QE, m)
+        pos = O.cold_beam_draws(g, rb, 7000.0, 1e10, mpw0, dt)
+        n = len(pos)
+        O.add_particles(g, Pb, ef, dt, QE, m, pos, np.tile([0.0, 0.0, 7000.0], (n, 1)),
+                        np.full(n, mpw0))
+    assert Pa.n == Pb.n > 10000
+    for a, b in zip(Pa.view(), Pb.view()):
+        assert np.array_equal(a, b)
+    assert ra.uniform() == rb.uniform()  # both streams consumed the same number of draws
+
+
+def test_sphere_history_400_steps_bit_exact():
+    """ch3/ver2/Main.cpp's main loop (:48-80) for 400 steps at dt 2e-7 and mpw0 200 with the
+    QN solver: per-step particle count, getRealCount / getMomentum / getKE, getPE,
+    World::steadyState and, from the step it holds, Species::updateAverages; den_ave and phi
+    at the end.  Every value bit-identical to the compiled reference's."""
+    G = load("hist_ch3.npz")
+    steps, dt, mpw0 = int(G["steps"]), float(G["dt"]), float(G["mpw0"])
+    g, oid, phi = sphere_history_world()
+    nv = g.node_volumes()
+    ef = O.compute_ef(g, phi)
+    rng = O.Rng(7)
+    P = O.Particles(600000)
+    steady = O.SteadyState()
+    den_ave, samples = g.zeros(), 0
+    nps, diags, pes, flags = [], [], [], []
+    for _ in range(len(G["np"])):
+        O.cold_beam_sample(g, P, rng, 7000.0, 1e10, mpw0, ef, dt, QE, 16 * AMU)
+        O.push_absorb(g, P, ef, dt, QE, 16 * AMU, (0, 0, 0.15), 0.05)
+        den = O.number_density(g, P.pos[:P.n], P.mpw[:P.n], nv)
+        rho = O.charge_density([QE], [den])
+        O.solve_qn(g, phi, rho, oid, 0, 1.5, 1e10)
+        ef = O.compute_ef(g, phi)
+        d = O.species_diag(P.vel[:P.n], P.mpw[:P.n], 16 * AMU)
+        pe = O.get_pe(g, ef, nv)
+        st = steady([d], pe)
+        if st:
+            samples = O.update_average(den_ave, den, samples)
+        nps.append(P.n)
+        diags.append(np.concatenate([[P.n], d]))
+        pes.append(pe)
+        flags.append(st)
+    assert len(nps) == steps + 1
+    assert np.array_equal(np.array(nps), G["np"])
+    assert np.array_equal(np.array(diags), G["diag"])
+    assert np.array_equal(np.array(pes), G["pe"])
+    assert np.array_equal(np.array(flags, dtype=np.int8), G["steady"])
+    assert 0 < samples < steps  # the averages started inside the run
+    assert np.array_equal(den_ave, G["den_ave"])
+    assert np.array_equal(phi, G["phi"])
+
+
+def test_config1_history_2000_steps_bit_exact():
+    """ch2/Main.cpp's loop on config 1 with the SOR tolerance at 1e-8, 2000 steps: the
+    per-step diagnostics of both species and getPE bit-identical to the compiled
+    reference's."""
+    G = load("hist_c1.npz")
+    tol = float(G["tol"])
+    x0, xm, dt = (-0.1, -0.1, -0.1), (0.1, 0.1, 0.1), 2e-10
+    g = O.OGrid(21, 21, 21, x0, xm)
+    nv = g.node_volumes()
+    rng = O.Rng(1)
+    ef = g.zeros(3)
+    Pi, Pe = O.Particles(80000), O.Particles(10000)
+    O.load_box(g, Pi, rng, x0, xm, 1e11, 80000, ef, dt, QE, 16 * AMU)
+    O.load_box(g, Pe, rng, (0, 0, 0), xm, 1e11, 10000, ef, dt, -QE, ME)
+    phi = g.zeros()
+    O.solve_sor_linear(g, phi, g.zeros(), 10000, tol)
+    ef = O.compute_ef(g, phi)
+    diag, pe = [], []
+    for _ in range(len(G["pe"])):
+        dens = []
+        for P, q, m in ((Pi, QE, 16 * AMU), (Pe, -QE, ME)):
+            O.push_reflect(g, P, ef, dt, q, m)
+            dens.append(O.number_density(g, P.pos[:P.n], P.mpw[:P.n], nv))
+        rho = O.charge_density([QE, -QE], dens)
+        O.solve_sor_linear(g, phi, rho, 10000, tol)
+        ef = O.compute_ef(g, phi)
+        diag.append(np.stack([np.concatenate([[P.n], O.species_diag(P.vel[:P.n], P.mpw[:P.n], m)])
+                              for P, m in ((Pi, 16 * AMU), (Pe, ME))]))
+        pe.append(O.get_pe(g, ef, nv))
+    assert np.array_equal(np.array(diag), G["diag"])
+    assert np.array_equal(np.array(pe), G["pe"])
