@@ -83,6 +83,9 @@ def _load(ch):
             "ref_steady_state": (C.c_int, [C.c_void_p]),
             "ref_update_averages": (None, [C.c_void_p, C.c_int]),
         })
+    if ch == "ch3":
+        sig["ref_output_fields"] = (None, [C.c_void_p])
+        sig["ref_output_diag"] = (None, [C.c_void_p])
     if ch == "mt":
         sig["ref_set_num_threads"] = (None, [C.c_void_p, C.c_int])
     for name, (res, args) in sig.items():
@@ -231,6 +234,15 @@ class RefSim:
 
     def update_averages(self, sp):
         self.L.ref_update_averages(self.h, sp)
+
+    def output_fields(self):
+        """Output::fields (ch3/ver2/Output.cpp:12-81) into results/ of the working directory."""
+        self.L.ref_output_fields(self.h)
+
+    def output_diag(self):
+        """Output::diagOutput (ch3/ver2/Output.cpp:97-132): runtime_diags.csv of the working
+        directory, opened by the first call of the process."""
+        self.L.ref_output_diag(self.h)
 
     def set_num_threads(self, n):
         self.L.ref_set_num_threads(self.h, n)

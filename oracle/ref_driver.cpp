@@ -26,6 +26,9 @@
 #if REF_CH != 2
 #include "Source.h"
 #endif
+#if REF_CH == 3
+#include "Output.h"
+#endif
 
 #define REF_API extern "C" __attribute__((visibility("default")))
 
@@ -257,6 +260,24 @@ REF_API int ref_steady_state(void* h) {
 }
 
 REF_API void ref_update_averages(void* h, int sp) { static_cast<RefSim*>(h)->species[sp].updateAverages(); }
+#endif
+
+#if REF_CH == 3
+// The reference's writers (ch3/ver2/Output.cpp) on the current state, into the working
+// directory: results/fields_<ts>.vti and one runtime_diags.csv row (the header on the first
+// call of the process); the CSV stream is flushed so that the row can be read back.
+namespace Output {
+extern std::ofstream f_diag;
+}
+REF_API void ref_output_fields(void* h) {
+    RefSim* s = static_cast<RefSim*>(h);
+    Output::fields(s->world, s->species);
+}
+REF_API void ref_output_diag(void* h) {
+    RefSim* s = static_cast<RefSim*>(h);
+    Output::diagOutput(s->world, s->species);
+    Output::f_diag.flush();
+}
 #endif
 
 #if REF_CH == 9

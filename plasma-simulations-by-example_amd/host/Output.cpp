@@ -56,27 +56,29 @@ void Output::screenOutput(World &world, vector<Species> &species) {
     cout << endl;
 }
 
-// runtime_diags.csv: ts,time,wall_time,{mp_count,real_count,px,py,pz,KE}.<sp>,PE,E_total,
-// flushed every 25 steps
+// runtime_diags.csv (ch3/ver2/Output.cpp:97-131): one row per call, in the stream's default
+// 6-digit format: ts, time, wall time, then per species the six columns below, then PE and
+// the total energy; the header is written by the first call, the rows are flushed every 25
+// steps.
 void Output::diagOutput(World &world, vector<Species> &species) {
+    static const char *const per_species[] = {"mp_count", "real_count", "px", "py", "pz", "KE"};
     if (!f_diag.is_open()) {
         f_diag.open("runtime_diags.csv");
         f_diag << "ts,time,wall_time";
         for (Species &sp : species)
-            f_diag << ",mp_count." << sp.name << ",real_count." << sp.name << ",px." << sp.name << ",py."
-                   << sp.name << ",pz." << sp.name << ",KE." << sp.name;
+            for (const char *col : per_species) f_diag << "," << col << "." << sp.name;
         f_diag << ",PE,E_total" << endl;
     }
     f_diag << world.getTs() << "," << world.getTime() << "," << world.getWallTime();
-    double tot_KE = 0;
+    double kinetic = 0;  // summed over the species in their order, then PE added
     for (Species &sp : species) {
-        double KE = sp.getKE();
-        tot_KE += KE;
-        double3 mom = sp.getMomentum();
-        f_diag << "," << sp.getNp() << "," << sp.getRealCount() << "," << mom[0] << "," << mom[1] << ","
-               << mom[2] << "," << KE;
+        const double ke = sp.getKE();
+        double3 p = sp.getMomentum();
+        f_diag << "," << sp.getNp();  // an integer column
+        for (double v : {sp.getRealCount(), p[0], p[1], p[2], ke}) f_diag << "," << v;
+        kinetic += ke;
     }
-    double PE = world.getPE();
-    f_diag << "," << PE << "," << (tot_KE + PE) << "\n";
+    const double pe = world.getPE();
+    f_diag << "," << pe << "," << (kinetic + pe) << "\n";
     if (world.getTs() % 25 == 0) f_diag.flush();
 }

@@ -2,6 +2,8 @@
 #include "World.h"
 
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -22,6 +24,10 @@ World::World(int ni, int nj, int nk)
       ef(ni, nj, nk), object_id(ni, nj, nk) {
     time_start = chrono::high_resolution_clock::now();
     pic_check(pic_world_create(ni, nj, nk, &handle), "pic_world_create");
+    // an unchanged Main.cpp never calls setPrecision: PICAMD_PRECISION=fp64 selects the FP64
+    // form for it (default: mixed)
+    const char *p = getenv("PICAMD_PRECISION");
+    if (p && strcmp(p, "fp64") == 0) precision = PIC_FP64;
 }
 
 World::~World() { pic_world_destroy(handle); }
@@ -76,26 +82,24 @@ bool World::inSphere(double3 x) {
     return m2 <= sphere_rad2;
 }
 
-// World::steadyState (ch3/ver2/World.cpp:129-158): relative change of mass, |p_z| and
-// energy all below 1e-3, from the device reductions
+// World::steadyState (ch3/ver2/World.cpp:129-158).  The totals over the species, from the
+// device reductions, are accumulated in the reference's order (energy starts from PE); the
+// state turns steady once each total has moved by less than 1e-3 of itself since the last
+// call, and stays steady without further reductions.
 bool World::steadyState(vector<Species> &species) {
     if (steady_state) return true;
-    double tot_mass = 0, tot_mom = 0, tot_en = getPE();
+    Totals now;
+    now.energy = getPE();
     for (Species &sp : species) {
-        tot_mass += sp.getRealCount();
-        double3 mom = sp.getMomentum();
-        tot_mom += std::abs(mom[2]);
-        tot_en += sp.getKE();
+        now.count += sp.getRealCount();
+        now.pz += std::abs(sp.getMomentum()[2]);
+        now.energy += sp.getKE();
     }
-    const double tol = 1e-3;
-    if (std::abs((tot_mass - last_mass) / tot_mass) < tol && std::abs((tot_mom - last_mom) / tot_mom) < tol &&
-        std::abs((tot_en - last_en) / tot_en) < tol) {
-        steady_state = true;
-        cout << "Steady state reached at time step " << ts << endl;
-    }
-    last_mass = tot_mass;
-    last_mom = tot_mom;
-    last_en = tot_en;
+    auto settled = [](double cur, double prev) { return std::abs((cur - prev) / cur) < 1e-3; };
+    steady_state = settled(now.count, previous.count) && settled(now.pz, previous.pz) &&
+                   settled(now.energy, previous.energy);
+    if (steady_state) cout << "Steady state reached at time step " << ts << endl;
+    previous = now;
     return steady_state;
 }
 

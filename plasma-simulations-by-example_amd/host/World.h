@@ -138,8 +138,12 @@ protected:
     double3 sphere_x0{0, 0, 0};
     double sphere_rad2 = 0;
     std::chrono::time_point<std::chrono::high_resolution_clock> time_start;
+    // steadyState(): the totals it compared last (real count, |z momentum|, PE + KE)
+    struct Totals {
+        double count = 0, pz = 0, energy = 0;
+    };
     bool steady_state = false;
-    double last_mass = 0, last_mom = 0, last_en = 0;
+    Totals previous;
 
     pic_world *handle = nullptr;
     int precision = PIC_MIXED;
