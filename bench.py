@@ -87,6 +87,10 @@ def parse():
                     help="multiplies every box's particle counts (1: the 1e5-4e6 range)")
     ap.add_argument("--no-worlds", action="store_true",
                     help="skip timing the same boxes as separate worlds")
+    ap.add_argument("--ensemble-eager", action="store_true",
+                    help="launch the ensemble step's kernels one by one instead of replaying its "
+                         "captured graphs (the same kernels; for profilers that cannot follow "
+                         "graph launches)")
     return ap.parse_args()
 
 
@@ -367,6 +371,9 @@ def ensemble_case(P, args):
         ens.load_box(i, 0, (-L / 2,) * 3, (L / 2,) * 3, b["den"], seed=1)
         ens.load_box(i, 1, (0.0,) * 3, (L / 2,) * 3, b["den"], seed=1)
     tuned = P.tuned_ensemble_params(ens)  # applied by the library at creation
+    if args.ensemble_eager:
+        ens.set_tuning(chunk_particles=tuned["chunk_particles"], window_nodes=tuned["window_nodes"],
+                       sor_threads=tuned["sor_threads"], sort_every=tuned["sort_every"], use_graph=False)
     t0 = time.time()
     ens.solve_initial(10000, ENS_TOL)
     log(f"[bench] ensemble: {len(bx)} boxes, {ens.stats()['particles']:.3g} particles, "
@@ -439,6 +446,17 @@ def ensemble_case(P, args):
                 sp.close()
             w.close()
     return out
+
+
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(args, grid, ef, T_ion):
@@ -529,8 +547,15 @@ def main():
             traffic = json.load(open(tfile))["hbm_bytes_per_particle"] * np_avg
         except (OSError, ValueError, KeyError):
             traffic = None
+    traffic_src = None
+    if traffic is not None:
+        tj = json.load(open(tfile))
+        traffic_src = (f"stored counter measurement: {tj['hbm_bytes_per_particle']:.2f} B/particle from "
+                       f"rocprofv3 FETCH_SIZE + WRITE_SIZE of bench.py --push-only, library of commit "
+                       f"{tj.get('measured_with_commit', '?')} (profiles/push_mixed_sphere257_traffic.json), "
+                       f"times this run's particles")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "k_push<PIC_MIXED, HALO> + k_push<PIC_MIXED, HALO_WIDE>", "bytes_per_launch": bytes_launch,
                 "kernel_ms": mixed["push_kernel_avg_ms"],
                 "copy_kernel_gbps": copy_gbps}
@@ -541,7 +566,7 @@ def main():
         if r is not None:
             res, n = r
             cpu = {"value": res["serial"]["value"], "unit": "particle-steps/s", "cores": 1,
-                   "kind": "reference",
+                   "kind": "reference", "cpu_model": cpu_model(),
                    "sample": f"reference ch3/ver2 Species::advance+computeNumberDensity, "
                              f"{n} particles x {args.cpu_steps} steps on the same "
                              f"{grid[0]}x{grid[1]}x{grid[2]} grid, sphere and E field",

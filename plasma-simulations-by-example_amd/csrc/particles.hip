@@ -747,13 +747,17 @@ __global__ void __launch_bounds__(PUSH_THREADS)
     __shared__ uint32_t sNb[3][27];
 
     // per-thread diagnostic sums in the particle arithmetic (a thread sees a few dozen
-    // particles of one tile), reduced in FP64 across the workgroup and the grid
+    // particles of one tile), reduced in FP64 across the workgroup and the grid.  The weight
+    // sum is FP64 in both forms: a run of equal FP32 weights rounds the same way at every
+    // addition, and the real count (and, through the frame velocity, the momentum and KE:
+    // sum w v = S + u W) would inherit that bias.
     using AT = typename std::conditional<PREC == PIC_MIXED, float, double>::type;
     AT acc[NDIAG_SLOTS] = {0, 0, 0, 0, 0, 0};
+    double accw = 0;
     const int tid = threadIdx.x;
     auto tally = [&](const PT &p) {
         AT vx = p.vx, vy = p.vy, vz = p.vz, w = p.w;
-        acc[0] += w;
+        accw += (double)p.w;
         acc[1] += vx * w;
         acc[2] += vy * w;
         acc[3] += vz * w;
@@ -1007,7 +1011,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
     // per-block partials of getRealCount, getMomentum, getKE and the particle count
 #pragma unroll
     for (int q = 0; q < NDIAG_SLOTS; q++) {
-        double v = warp_sum((double)acc[q]);
+        double v = warp_sum(q == 0 ? accw : (double)acc[q]);
         if ((tid & 63) == 0) sRed[q][tid >> 6] = v;
     }
     __syncthreads();

@@ -30,6 +30,20 @@ def test_mt19937_known_answer():
     assert r.next_u32() == 4123659995  # the value the C++ standard fixes for mt19937
 
 
+def test_philox4x32_10_known_answers():
+    """The oracle's Philox4x32-10 (the counter-based generator of the device injection and
+    of the differentiable run) against the published known-answer vectors of the Random123
+    library (Salmon et al., SC11): zero counter and key, all-ones, and the pi-digit case."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in kat:
+        assert O.philox4x32_10(ctr, key) == list(want), (ctr, key)
+
+
 def test_node_volumes_closed_form():
     g = grid()
     nv = g.node_volumes()
