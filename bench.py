@@ -358,7 +358,7 @@ ENS_TOL = 1e-4  # ch2/Main.cpp:38
 
 
 def ensemble_case(P, args):
-    """Config-4 ensemble: every box's ch2 step (push + gather + deposit of both species,
+    """The 64-box ensemble: every box's ch2 step (push + gather + deposit of both species,
     density, rho, SOR, computeEF) in one hipGraph replay, against the same boxes run as
     separate worlds through the per-world entry points."""
     spec = [(16 * AMU, QE), (9.10938215e-31, -QE)]

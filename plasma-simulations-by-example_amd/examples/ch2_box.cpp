@@ -1,4 +1,4 @@
-// Grounded box (config 1): O+ ions over the box and electrons over one octant, reflecting
+// Grounded box (21^3 nodes, 80k O+ and 10k e-): O+ ions over the box and electrons over one octant, reflecting
 // walls, linear SOR.  The call sequence is that of ch2/Main.cpp:14-75, written against the
 // drop-in classes; the particle load draws from a seeded rnd, so the initial state is the
 // reference's for the same seed.

@@ -8,13 +8,13 @@ only.  Rerun with:  python tests/golden/make_golden.py
 Files
   rng_mt19937.npz      reference Rnd stream after reseeding (seeds 1 and 5489); first
                        particles of Species::loadParticlesBox (ch2)
-  c1_ch2.npz           config 1 (ch2 grounded box, 21^3, 80k O+ / 10k e-, seed 1,
+  box_ch2.npz          the ch2 grounded box (21^3, 80k O+ / 10k e-, seed 1,
                        dt 2e-10, SOR w=1.4 tol 1e-2): initial solve + 10 steps; fields, a
                        particle subset, per-step diagnostics
   qs_ch2.npz           Species::loadParticlesBoxQS on an 11^3 particle lattice
   sphere_ch3.npz       ch3/ver2 sphere (13x13x25), cold beam (seed 7), 12 steps for each
                        solver type GS / PCG / QN
-  hist_c1.npz          config 1 as c1_ch2 but SOR tol 1e-8, 2000 steps: per-step diagnostics
+  hist_box.npz         the grounded box as box_ch2 but SOR tol 1e-8, 2000 steps: per-step diagnostics
                        (getRealCount, getMomentum, getKE of both species) and getPE
   hist_ch3.npz         ch3/ver2/Main.cpp's sphere run (21x21x41, -100 V, cold beam 7 km/s,
                        seed 7) with the QN solver, dt 2e-7, mpw0 200, 400 steps: per-step
@@ -36,7 +36,7 @@ QE = 1.602176565e-19
 AMU = 1.660538921e-27
 ME = 9.10938215e-31
 
-C1 = dict(n=(21, 21, 21), x0=(-0.1, -0.1, -0.1), xm=(0.1, 0.1, 0.1), dt=2e-10)
+BOX = dict(n=(21, 21, 21), x0=(-0.1, -0.1, -0.1), xm=(0.1, 0.1, 0.1), dt=2e-10)
 SPH = dict(n=(13, 13, 25), x0=(-0.1, -0.1, 0.0), xm=(0.1, 0.1, 0.4), dt=1e-7)
 
 
@@ -47,23 +47,23 @@ def gen_rng():
         sim.reseed(seed)
         out[f"u_seed{seed}"] = np.array([sim.L.ref_rnd() for _ in range(64)])
     sim.close()
-    sim = R.RefSim("ch2", *C1["n"], C1["x0"], C1["xm"], C1["dt"], 10)
+    sim = R.RefSim("ch2", *BOX["n"], BOX["x0"], BOX["xm"], BOX["dt"], 10)
     sp = sim.add_species("O+", 16 * AMU, QE)
     sim.reseed(1)
-    sim.load_box(sp, C1["x0"], C1["xm"], 1e11, 200)
+    sim.load_box(sp, BOX["x0"], BOX["xm"], 1e11, 200)
     pos, vel, mpw = sim.particles(sp)
     out.update(load_pos=pos, load_mpw=mpw)
     sim.close()
     np.savez_compressed(os.path.join(HERE, "rng_mt19937.npz"), **out)
 
 
-def gen_c1():
-    sim = R.RefSim("ch2", *C1["n"], C1["x0"], C1["xm"], C1["dt"], 10)
+def gen_box():
+    sim = R.RefSim("ch2", *BOX["n"], BOX["x0"], BOX["xm"], BOX["dt"], 10)
     i = sim.add_species("O+", 16 * AMU, QE)
     e = sim.add_species("e-", ME, -QE)
     sim.reseed(1)
-    sim.load_box(i, C1["x0"], C1["xm"], 1e11, 80000)
-    sim.load_box(e, (0.0, 0.0, 0.0), C1["xm"], 1e11, 10000)
+    sim.load_box(i, BOX["x0"], BOX["xm"], 1e11, 80000)
+    sim.load_box(e, (0.0, 0.0, 0.0), BOX["xm"], 1e11, 10000)
     sim.make_solver(0, 10000, 1e-2)
     sim.solve()
     sim.compute_ef()
@@ -80,7 +80,7 @@ def gen_c1():
     pi, vi, _ = sim.particles(i)
     pe_, ve, _ = sim.particles(e)
     np.savez_compressed(
-        os.path.join(HERE, "c1_ch2.npz"),
+        os.path.join(HERE, "box_ch2.npz"),
         steps=len(pe), phi=sim.field(R.PHI), rho=sim.field(R.RHO), ef=sim.field(R.EF),
         den_i=sim.den(i), den_e=sim.den(e), ion_idx=np.arange(0, 80000, 97),
         ion_pos=pi[::97], ion_vel=vi[::97], ele_idx=np.arange(0, 10000, 13), ele_pos=pe_[::13],
@@ -138,13 +138,13 @@ def gen_sphere():
     np.savez_compressed(os.path.join(HERE, "sphere_ch3.npz"), **out)
 
 
-def gen_hist_c1(steps=2000, tol=1e-8):
-    sim = R.RefSim("ch2", *C1["n"], C1["x0"], C1["xm"], C1["dt"], steps)
+def gen_hist_box(steps=2000, tol=1e-8):
+    sim = R.RefSim("ch2", *BOX["n"], BOX["x0"], BOX["xm"], BOX["dt"], steps)
     i = sim.add_species("O+", 16 * AMU, QE)
     e = sim.add_species("e-", ME, -QE)
     sim.reseed(1)
-    sim.load_box(i, C1["x0"], C1["xm"], 1e11, 80000)
-    sim.load_box(e, (0.0, 0.0, 0.0), C1["xm"], 1e11, 10000)
+    sim.load_box(i, BOX["x0"], BOX["xm"], 1e11, 80000)
+    sim.load_box(e, (0.0, 0.0, 0.0), BOX["xm"], 1e11, 10000)
     sim.make_solver(0, 10000, tol)
     sim.solve()
     sim.compute_ef()
@@ -158,7 +158,7 @@ def gen_hist_c1(steps=2000, tol=1e-8):
         sim.compute_ef()
         diag.append(np.stack([sim.species_diag(i), sim.species_diag(e)]))
         pe.append(sim.pe())
-    np.savez_compressed(os.path.join(HERE, "hist_c1.npz"), diag=np.array(diag), pe=np.array(pe),
+    np.savez_compressed(os.path.join(HERE, "hist_box.npz"), diag=np.array(diag), pe=np.array(pe),
                         tol=tol, steps=steps)
     sim.close()
 
@@ -205,9 +205,9 @@ if __name__ == "__main__":
         R.build()
     gen_rng()
     gen_qs()
-    gen_c1()
+    gen_box()
     gen_sphere()
-    gen_hist_c1()
+    gen_hist_box()
     gen_hist_ch3()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

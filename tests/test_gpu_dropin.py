@@ -1,4 +1,4 @@
-"""The C++ drop-in classes run the reference's main-loop call sequence on the GPU (config 1
+"""The C++ drop-in classes run the reference's main-loop call sequence on the GPU (the grounded box
 and the ch3 sphere) and agree with the oracle replaying the same seeded run.
 
 With the solver tolerance tight (1e-8 here) the red-black and lexicographic SOR iterates
@@ -35,7 +35,7 @@ def run_example(tmp_path, name, *args):
     return r.stdout, rows
 
 
-def oracle_config1(num_ts, tol):
+def oracle_grounded_box(num_ts, tol):
     x0, xm, dt = (-0.1, -0.1, -0.1), (0.1, 0.1, 0.1), 2e-10
     g = O.OGrid(21, 21, 21, x0, xm)
     nv = g.node_volumes()
@@ -67,7 +67,7 @@ def test_ch2_box_dropin_matches_oracle(tmp_path, fp64, rtol):
     num_ts = 10
     stdout, rows = run_example(tmp_path, "ch2_box", num_ts, 1, fp64, 1e-8)
     assert "Simulation took" in stdout
-    ref = oracle_config1(num_ts, 1e-8)
+    ref = oracle_grounded_box(num_ts, 1e-8)
     last = rows[-1]
     ts = int(last["ts"])
     assert ts == num_ts

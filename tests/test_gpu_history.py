@@ -1,7 +1,7 @@
 """Long-run histories on the GPU against the compiled reference's own (tests/golden/hist_*.npz,
 written by oracle/_ref; the oracle replays them bit for bit on the CPU).
 
-Both runs start from the reference's seeded inputs: config 1's particles come from the same
+Both runs start from the reference's seeded inputs: the grounded box's particles come from the same
 mt19937 draws (Species::loadParticlesBox, ch2/Species.cpp:74-97), and the sphere run's beam
 from the same draws of ColdBeamSource::sample (ch3/ver2/Source.cpp:4-27), handed to the GPU
 through addParticle, which rewinds them with the GPU's live field.
@@ -38,7 +38,7 @@ def rel(a, b):
 def momentum_scale(diag, mass):
     """Per-species scale of the momentum columns: sqrt(2 m KE W), the largest |sum m w v| that
     the real count W and the kinetic energy KE allow (Cauchy-Schwarz).  A net momentum that
-    cancels to near zero (config 1's ions) is judged against it, not against its own size."""
+    cancels to near zero (the grounded box's ions) is measured against it, not against its own size."""
     return np.sqrt(2 * mass * np.abs(diag[..., 5]).max(axis=0) * np.abs(diag[..., 1]).max(axis=0))
 
 
@@ -146,8 +146,8 @@ def test_sphere_history_mixed_within_budget():
         assert d[k] <= 4 * c[k] + 1e-4 and d[k] <= 3e-3, (k, d[k], c[k])
 
 
-def run_config1_history(prec, G):
-    """ch2/Main.cpp:14-75 on config 1 with the SOR tolerance of the golden run."""
+def run_box_history(prec, G):
+    """ch2/Main.cpp:14-75 on the grounded box with the SOR tolerance of the golden run."""
     tol = float(G["tol"])
     x0, xm, dt = (-0.1, -0.1, -0.1), (0.1, 0.1, 0.1), 2e-10
     g = O.OGrid(21, 21, 21, x0, xm)
@@ -178,15 +178,15 @@ def run_config1_history(prec, G):
 
 
 # (diagnostics, PE): largest deviation over the run relative to the quantity's scale
-CONFIG1_TOL = {P.FP64: (1e-9, 1e-9), P.MIXED: (1e-4, 1e-4)}
+BOX_TOL = {P.FP64: (1e-9, 1e-9), P.MIXED: (1e-4, 1e-4)}
 
 
 @pytest.mark.parametrize("prec", [P.FP64, P.MIXED], ids=["fp64", "mixed"])
-def test_config1_history_matches_reference(prec):
-    G = load("hist_c1.npz")
-    print(f"config-1 history {'fp64' if prec == P.FP64 else 'mixed'}: {len(G['pe'])} steps",
+def test_grounded_box_history_matches_reference(prec):
+    G = load("hist_box.npz")
+    print(f"grounded-box history {'fp64' if prec == P.FP64 else 'mixed'}: {len(G['pe'])} steps",
           flush=True)
-    diags, pes = run_config1_history(prec, G)
+    diags, pes = run_box_history(prec, G)
     assert np.array_equal(diags[:, :, 0], G["diag"][:, :, 0])  # reflecting walls keep every one
     dev = diag_deviation(diags, G["diag"], [16 * AMU, ME])
     dev_pe = np.abs(pes - G["pe"]) / np.abs(G["pe"]).max()
@@ -195,5 +195,5 @@ def test_config1_history_matches_reference(prec):
     growth = [float(max(dev[i * q:(i + 1) * q].max(), dev_pe[i * q:(i + 1) * q].max()))
               for i in range(4)]
     print(f"  diag {dev.max():.3e}  pe {dev_pe.max():.3e}  by quarter {growth}", flush=True)
-    t_diag, t_pe = CONFIG1_TOL[prec]
+    t_diag, t_pe = BOX_TOL[prec]
     assert dev.max() <= t_diag and dev_pe.max() <= t_pe

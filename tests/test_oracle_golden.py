@@ -48,9 +48,9 @@ def test_load_box_qs_matches_reference():
     assert np.array_equal(mpw, G["mpw"])
 
 
-def test_config1_ten_steps_bit_exact():
-    """ch2/Main.cpp loop on config 1: load, solve, then advance/deposit/rho/SOR/EF."""
-    G = load("c1_ch2.npz")
+def test_grounded_box_ten_steps_bit_exact():
+    """ch2/Main.cpp loop on the grounded box (21^3, 80k O+ / 10k e-): load, solve, then advance/deposit/rho/SOR/EF."""
+    G = load("box_ch2.npz")
     x0, xm, dt = (-0.1, -0.1, -0.1), (0.1, 0.1, 0.1), 2e-10
     g = O.OGrid(21, 21, 21, x0, xm)
     nv = g.node_volumes()
@@ -214,11 +214,11 @@ def test_sphere_history_400_steps_bit_exact():
     assert np.array_equal(phi, G["phi"])
 
 
-def test_config1_history_2000_steps_bit_exact():
-    """ch2/Main.cpp's loop on config 1 with the SOR tolerance at 1e-8, 2000 steps: the
+def test_grounded_box_history_2000_steps_bit_exact():
+    """ch2/Main.cpp's loop on the grounded box with the SOR tolerance at 1e-8, 2000 steps: the
     per-step diagnostics of both species and getPE bit-identical to the compiled
     reference's."""
-    G = load("hist_c1.npz")
+    G = load("hist_box.npz")
     tol = float(G["tol"])
     x0, xm, dt = (-0.1, -0.1, -0.1), (0.1, 0.1, 0.1), 2e-10
     g = O.OGrid(21, 21, 21, x0, xm)
