@@ -433,10 +433,11 @@ __global__ void __launch_bounds__(256)
 }
 
 // the deferred y update of the last completed iteration
-__global__ void __launch_bounds__(256) k_pcg_finish(long nn, const double *d, double *y, const DevScalars *sc) {
+template <typename TD>
+__global__ void __launch_bounds__(256) k_pcg_finish(long nn, const TD *d, double *y, const DevScalars *sc) {
     long u = xcd_block() * blockDim.x + threadIdx.x;
     double lam = sc->red[7];
-    if (u < nn) y[u] = y[u] + lam * d[u];
+    if (u < nn) y[u] = y[u] + lam * (double)d[u];
 }
 
 // scalar steps (one workgroup): red[0] alpha, red[1] beta, red[2] g.g, red[3] alpha',
@@ -686,12 +687,16 @@ __global__ void __launch_bounds__(256)
 //   pass B:  g += (a/b) z, partials g.g (convergence: RMS(g) < tol)
 //   s = V(g), with the partials of g.s from its last sweep:  mu = a'/a
 // Scalars: red[0] a, red[1] b, red[4] RMS(g), red[6] mu, red[7] lam = a/b.
+// The direction d is stored in FP32 (s, of which it is a running combination, is FP32 from
+// the V-cycle already).  z = J d is formed from the stored, rounded d, and y advances by the
+// same stored d, so the FP64 recurrence g = J y - F stays exact; the rounding only perturbs
+// the direction, as a slightly different preconditioner would.
 
 // FIRST: the first pass of a solve from y = 0, d = 0 (mu = lam = 0), neither of them read
 template <bool FIRST>
 __global__ void __launch_bounds__(256)
-    k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const float *sv, const double *d_old,
-              double *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
+    k_mgpcg_a(PcgParams Q, const uint8_t *mask, const double *jr, const float *sv, const float *d_old,
+              float *d_new, double *y, double *z, double *partials, const DevScalars *sc) {
     // d_old and s are zero off the regular nodes and every regular node is interior, so the
     // six neighbours are read without waiting for the mask (non-regular ones add 0)
     __shared__ double sh[4];
@@ -708,17 +713,20 @@ __global__ void __launch_bounds__(256)
         if (in) {
             long sj = g.ni, sk = (long)g.ni * g.nj;
             uint8_t m = mask[u];
-            auto dold = [&](long v) { return FIRST ? 0.0 : d_old[v]; };
+            auto dold = [&](long v) { return FIRST ? 0.0 : (double)d_old[v]; };
+            // the new direction at node v as stored (one fma, then FP32): every thread that
+            // forms it gets the same value, so z below is J applied to the stored d
+            auto dnew = [&](long v) { return (double)(float)fma(mu, dold(v), -(double)sv[v]); };
             double jru = jr[u], dou = dold(u), yu = FIRST ? 0.0 : y[u];
-            double n0 = mu * dold(u - sk) - (double)sv[u - sk];
-            double n1 = mu * dold(u - sj) - (double)sv[u - sj];
-            double n2 = mu * dold(u - 1) - (double)sv[u - 1];
-            double n3 = mu * dold(u + 1) - (double)sv[u + 1];
-            double n4 = mu * dold(u + sj) - (double)sv[u + sj];
-            double n5 = mu * dold(u + sk) - (double)sv[u + sk];
+            double n0 = dnew(u - sk);
+            double n1 = dnew(u - sj);
+            double n2 = dnew(u - 1);
+            double n3 = dnew(u + 1);
+            double n4 = dnew(u + sj);
+            double n5 = dnew(u + sk);
             if (m & 64) {
                 y[u] = yu + lam * dou;
-                dc = mu * dou - (double)sv[u];
+                dc = dnew(u);
                 // slot order of the reference rows: -z, -y, -x, centre, +x, +y, +z
                 zz = Q.idz2 * n0;
                 zz += Q.idy2 * n1;
@@ -729,7 +737,7 @@ __global__ void __launch_bounds__(256)
                 zz += Q.idz2 * n5;
             }
         }
-        d_new[u] = dc;
+        d_new[u] = (float)dc;
         z[u] = zz;
         dz = dc * zz;
     }
@@ -821,13 +829,13 @@ __global__ void __launch_bounds__(256)
 // d_pend (nullable): the CG's last deferred update y += lam d is applied here, on the values
 // read, instead of by k_pcg_finish (lam = sc->red[7]; the same operations, so the same y)
 __global__ void __launch_bounds__(256)
-    k_nrs_update(PcgParams Q, const uint8_t *oid, double *x, const double *y, const double *d_pend,
+    k_nrs_update(PcgParams Q, const uint8_t *oid, double *x, const double *y, const float *d_pend,
                  const DevScalars *sc, double *partials) {
     __shared__ double sh[4];
     long u = xcd_block() * blockDim.x + threadIdx.x;
     const Grid &g = Q.g;
     const double lam = d_pend ? sc->red[7] : 0.0;
-    auto yv = [&](long v) { return d_pend ? y[v] + lam * d_pend[v] : y[v]; };
+    auto yv = [&](long v) { return d_pend ? y[v] + lam * (double)d_pend[v] : y[v]; };
     double y2 = 0;
     if (u < g.nn) {
         int i, j, k;
@@ -988,7 +996,7 @@ static int pcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *i
     }
     // d of the last completed iteration sits in dbuf[iters & 1]
     int done_its = w->scal_host->iters;
-    k_pcg_finish<<<nb8, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    k_pcg_finish<double><<<nb8, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
     PIC_LAUNCH_CHECK();
     *its += done_its;
     *converged = conv;
@@ -1121,7 +1129,7 @@ static int pcgs_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *
         }
     }
     int done_its = w->scal_host->iters;
-    k_pcg_finish<<<nb8, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+    k_pcg_finish<double><<<nb8, 256, 0, w->stream>>>(nn, dbuf[done_its & 1], s->y, w->scal);
     PIC_LAUNCH_CHECK();
     *its += done_its;
     *converged = conv;
@@ -1164,7 +1172,7 @@ void mg_destroy(MgHierarchy *H);
 // solvePCGLinear on the reduced system with one V-cycle as the preconditioner
 // *d_pend: when converged, the direction whose update y += lam d the caller's k_nrs_update
 // applies (null when y is final here)
-static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *its, const double **d_pend) {
+static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int *its, const float **d_pend) {
     pic_world *w = s->w;
     long nn = Q.g.nn;
     unsigned nb = nbk(nn);
@@ -1174,7 +1182,8 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     if (rc) return rc;
     double *folded = w->partials + 4 * (size_t)w->partials_cap;
     double nu = (double)(int)nn;
-    double *dbuf[2] = {s->d, s->z2};
+    // the FP32 directions, in the storage of the FP64 vectors the Jacobi-preconditioned CG uses
+    float *dbuf[2] = {reinterpret_cast<float *>(s->d), reinterpret_cast<float *>(s->z2)};
     hipStream_t st = w->stream;
     k_reset_flags<<<1, 1, 0, st>>>(w->scal);
     // s = V(g) in FP32; the V-cycle's last sweep also writes the partials of g.s
@@ -1219,7 +1228,7 @@ static int mgpcg_linear(pic_solver *s, const PcgParams &Q, bool *converged, int 
     if (done_its > 0 && conv)
         *d_pend = dbuf[done_its & 1];
     else if (done_its > 0)  // the Gauss-Seidel fallback continues from y
-        k_pcg_finish<<<nb8, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
+        k_pcg_finish<float><<<nb8, 256, 0, st>>>(nn, dbuf[done_its & 1], s->y, w->scal);
     else  // converged at the start (or no iteration allowed): the correction is 0
         k_fill<<<nbk(nn), 256, 0, st>>>(nn, s->y, 0.0);
     PIC_LAUNCH_CHECK();
@@ -1271,7 +1280,7 @@ static int solve_nrpcg_sym(pic_solver *s, double *residual, int *iters) {
         if (!use_mg && it > 0) k_fill<<<nb, 256, 0, w->stream>>>(nn, s->y, 0.0);
         PIC_LAUNCH_CHECK();
         bool pc = false;
-        const double *d_pend = nullptr;
+        const float *d_pend = nullptr;
         if (use_mg) {
             rc = mg_setup(s->mg, Q, s->boltz, w->geom_stamp, w->stream);
             if (rc) return rc;
@@ -1353,10 +1362,10 @@ int solve_linearized(pic_solver *s, const double *rhs, double *lam, double tol, 
     if (use_mg) {
         rc = mg_setup(s->mg, Q, s->boltz, w->geom_stamp, w->stream);
         if (rc) return rc;
-        const double *d_pend = nullptr;
+        const float *d_pend = nullptr;
         rc = mgpcg_linear(s, Q, &conv, &its, &d_pend);
         if (rc) return rc;
-        if (d_pend) k_pcg_finish<<<nb8, 256, 0, w->stream>>>(nn, d_pend, s->y, w->scal);
+        if (d_pend) k_pcg_finish<float><<<nb8, 256, 0, w->stream>>>(nn, d_pend, s->y, w->scal);
     } else {
         k_fill<<<nbk(nn), 256, 0, w->stream>>>(nn, s->y, 0.0);
         rc = pcgs_linear(s, Q, &conv, &its);
