@@ -257,13 +257,18 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     # untimed continuation to the comparison horizon; diagnostics of every step
     hist = []
     late_push = []  # push time of each continuation step (the wake fills in over ~100 steps)
+    late_kernel = []  # the push kernel's HIP-event time of each continuation step
+    late_np = []  # particles pushed at each continuation step
     w.set_timing(True)
     w.timers(reset=True)
     for i in range(warmup + steps, max(horizon, warmup + steps)):
+        late_np.append(ions.np())
         ok, its = step()
         if not ok:
             log(f"[bench]   step {i}: Newton did not converge")
-        late_push.append(w.timers(reset=True)["push"][0])
+        tl = w.timers(reset=True)
+        late_push.append(tl["push"][0])
+        late_kernel.append(tl["push_kernel"][0] / max(tl["push_kernel"][1], 1))
         d = ions.diag()  # getRealCount, getMomentum (3), getKE
         hist.append([float(ions.np()), d[0], d[3], d[4], w.get_pe()])
         if (i + 1) % 20 == 0:
@@ -272,6 +277,8 @@ def run_gpu(P, grid, nparticles, precision, steps, warmup, T_ion, want_fields=Fa
     w.set_timing(False)
     out["history"] = np.array(hist)
     out["late_push_ms"] = late_push
+    out["late_kernel_ms"] = late_kernel
+    out["late_np"] = late_np
     out["steps_total"] = max(horizon, warmup + steps)
     if want_fields:
         out["phi"] = w.field(P.PHI)
@@ -630,6 +637,16 @@ def main():
                 "steps": f"{mixed['steps_total'] - 19}-{mixed['steps_total']}",
                 "mixed": float(np.mean(mixed["late_push_ms"][-20:])),
                 "fp64": float(np.mean(fp64["late_push_ms"][-20:]))}
+            # the same roofline as above over the last 20 steps of the continuation
+            np_late = float(np.mean(mixed["late_np"][-20:]))
+            ms_late = float(np.mean(mixed["late_kernel_ms"][-20:]))
+            b_late = np_late * BYTES_PER_PARTICLE_STEP + mixed["nn"] * BYTES_PER_NODE_E
+            a_late = b_late / (ms_late * 1e-3)
+            line["late_roofline"] = {
+                "steps": line["late_push_ms"]["steps"], "bound": "hbm", "achieved": a_late / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": a_late / HBM_PEAK,
+                "bytes_per_launch": b_late, "kernel_ms": ms_late, "particles": np_late,
+                "particle_steps_per_s": np_late / (ms_late * 1e-3)}
     if not args.no_adjoint:
         line["adjoint"] = adjoint_case(P, args)
     if not args.no_ensemble:
