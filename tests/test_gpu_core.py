@@ -6,7 +6,11 @@ rounding (computeEF and node volumes bit-exact; deposits to 1e-13 relative, sinc
 summation order differs).  PIC_MIXED species store the position as cell + FP32 offset and push in
 FP32: positions within 2e-4 cells (RMS 1e-5 cells), velocities within 2e-5 of the velocity
 scale, densities within 1e-5 relative L2 over a 10-step horizon -- far inside the 1e-4
-relative field-error budget.
+relative field-error budget.  Where the RMS bound comes from: an FP32 velocity carries a
+relative rounding of about 6e-8 per step, and at up to 3 cells per step (3e7 m/s, 1e-9 s,
+1 cm cells) that moves the position by about 2e-7 cells per step; over 10 steps the errors
+add to an RMS of about 2e-6 cells, which is what the runs measure (2.1e-6 at most), and the
+bound leaves a factor of 5.
 """
 import numpy as np
 import pytest
