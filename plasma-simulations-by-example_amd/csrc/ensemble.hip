@@ -13,8 +13,10 @@
 //   - k_ens_finish: den = raw / node_vol per species (clearing raw), rho = sum q_s den_s;
 //   - k_ens_sor: one workgroup per box whose phi fits in LDS runs that box's whole SOR solve
 //     (sweeps, residual, stopping test); the larger boxes are swept by one launch per
-//     half-sweep over all of them (k_ens_sweep), chunks of 25 iterations and a per-box
-//     residual check (k_ens_resid, k_ens_check), converged boxes skipping at once;
+//     iteration over all of them (k_ens_sor_block: blocks of 8^3 nodes, both half-sweeps
+//     from an LDS window, into a second phi buffer; or k_ens_sweep, one launch per
+//     half-sweep), chunks of 25 iterations and a per-box residual check (k_ens_resid,
+//     k_ens_check), converged boxes skipping at once;
 //   - k_ens_ef: computeEF of every box.
 // The step is captured in three hipGraphs (push + density + the first SOR iteration, a chunk
 // of 25 SOR iterations, computeEF) and replayed; the host reads the count of converged boxes
@@ -461,9 +463,11 @@ struct EnsSweepArgs {
     const double *rho;
     const int *done;
     double w;
+    double *copy = nullptr;  // k_ens_resid: also copies the interior of phi here
 };
 
 __global__ void __launch_bounds__(256) k_ens_sweep(EnsSweepArgs S, int color) {
+#pragma clang fp contract(off)
     const int bi = S.blk_box[blockIdx.x];
     if (S.done[bi]) return;
     const EnsBox B = S.box[bi];
@@ -486,6 +490,99 @@ __global__ void __launch_bounds__(256) k_ens_sweep(EnsSweepArgs S, int color) {
                       idz2 * (phi[u - sk] + phi[u + sk])) /
                      denom;
     phi[u] = phi[u] + S.w * (phi_new - phi[u]);
+}
+
+// ---- the same SOR, one iteration per launch ---------------------------------------------------
+// A launch of k_ens_sweep costs about 13 us however few nodes it updates, and the largest box
+// needs about 1400 iterations, most of them after the smaller boxes have converged.  Here a
+// workgroup takes a block of ENS_SB^3 interior nodes of one box and its window with a T-node
+// rim in LDS, runs T half-sweeps there (red first), each on the nodes the later ones still
+// need (the block grown by T - 1 - h nodes), and writes the block's nodes to the other phi
+// buffer.  Every node gets the same updates, from the same values, as from T launches of
+// k_ens_sweep; the rim's nodes are recomputed by each block that needs them.  T = 2 (one
+// iteration) with 8^3 blocks measured best: larger blocks leave the largest box too few
+// workgroups, and T = 4 loads a window 2.4 times the block's nodes.
+constexpr int ENS_SB = 8;
+constexpr int ENS_T = 2;
+
+struct EnsBlockArgs {
+    const EnsBox *box;
+    const int4 *blk;     // per workgroup: box, first interior i, j, k of its block
+    const double *src;   // phi before this launch
+    double *dst;         // phi after it (the other buffer)
+    const double *rho;
+    const int *done;
+    double w;
+};
+
+template <int T>
+__global__ void __launch_bounds__(256) k_ens_sor_block(EnsBlockArgs S) {
+#pragma clang fp contract(off)
+    extern __shared__ double sp[];
+    constexpr int W = ENS_SB + 2 * T;
+    const int4 bk = S.blk[blockIdx.x];
+    if (S.done[bk.x]) return;
+    const EnsBox B = S.box[bk.x];
+    const long sj = B.ni, sk = (long)B.ni * B.nj;
+    const double *src = S.src + B.node0;
+    double *dst = S.dst + B.node0;
+    const double *rho = S.rho + B.node0;
+    const int i0 = bk.y - T, j0 = bk.z - T, k0 = bk.w - T;  // grid index of window node 0
+    const double idx2 = 1.0 / (B.dh[0] * B.dh[0]), idy2 = 1.0 / (B.dh[1] * B.dh[1]),
+                 idz2 = 1.0 / (B.dh[2] * B.dh[2]);
+    const double denom = 2 * idx2 + 2 * idy2 + 2 * idz2;
+    // the window: every load of a thread issued before the first LDS store
+    constexpr int NW = W * W * W, PER = (NW + 255) / 256;
+    double v[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int n = threadIdx.x + 256 * t;
+        const int a = n % W, b = (n / W) % W, c = n / (W * W);
+        const int gi = i0 + a, gj = j0 + b, gk = k0 + c;
+        const bool in = n < NW && gi >= 0 && gi < B.ni && gj >= 0 && gj < B.nj && gk >= 0 && gk < B.nk;
+        v[t] = in ? src[gi + sj * gj + sk * gk] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int n = threadIdx.x + 256 * t;
+        if (n < NW) sp[n] = v[t];
+    }
+    __syncthreads();
+    for (int h = 0; h < T; h++) {
+        const int color = h & 1, e = T - 1 - h;
+        // interior nodes of the block grown by e: [a, b) per axis
+        const int ia = max(bk.y - e, 1), ib = min(bk.y + ENS_SB + e, B.ni - 1);
+        const int ja = max(bk.z - e, 1), jb = min(bk.z + ENS_SB + e, B.nj - 1);
+        const int ka = max(bk.w - e, 1), kb = min(bk.w + ENS_SB + e, B.nk - 1);
+        const int mi = ib - ia, mj = jb - ja, mk = kb - ka;
+        const int half = (mi + 1) / 2;
+        const int cnt = (mi > 0 && mj > 0 && mk > 0) ? half * mj * mk : 0;
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+            const int r = q / half, s = q - r * half;
+            const int gj = ja + r % mj, gk = ka + r / mj;
+            const int gi = ia + 2 * s + (((ia + gj + gk) & 1) != color ? 1 : 0);
+            if (gi >= ib) continue;
+            const int l = (gi - i0) + W * ((gj - j0) + W * (gk - k0));
+            const double phi_new = (rho[gi + sj * gj + sk * gk] / EPS_0 + idx2 * (sp[l - 1] + sp[l + 1]) +
+                                    idy2 * (sp[l - W] + sp[l + W]) + idz2 * (sp[l - W * W] + sp[l + W * W])) /
+                                   denom;
+            sp[l] = sp[l] + S.w * (phi_new - sp[l]);
+        }
+        __syncthreads();
+    }
+    const int mi = min(bk.y + ENS_SB, B.ni - 1) - bk.y, mj = min(bk.z + ENS_SB, B.nj - 1) - bk.z,
+              mk = min(bk.w + ENS_SB, B.nk - 1) - bk.w;
+    for (int q = threadIdx.x; q < mi * mj * mk; q += blockDim.x) {
+        const int a = q % mi, r = q / mi, b = r % mj, c = r / mj;
+        const int gi = bk.y + a, gj = bk.z + b, gk = bk.w + c;
+        dst[gi + sj * gj + sk * gk] = sp[(gi - i0) + W * ((gj - j0) + W * (gk - k0))];
+    }
+}
+
+// LDS bytes of k_ens_sor_block<T>: its window of phi
+inline size_t sor_block_lds(int T) {
+    const size_t w = ENS_SB + 2 * T;
+    return sizeof(double) * w * w * w;
 }
 
 // per-workgroup partial sums of R^2 over the interior nodes (ch2/PotentialSolver.cpp:48-58)
@@ -515,6 +612,7 @@ __global__ void __launch_bounds__(256) k_ens_resid(EnsSweepArgs S, double *part)
         double R = -phi[u] * denom + rho[u] / EPS_0 + idx2 * (phi[u - 1] + phi[u + 1]) +
                    idy2 * (phi[u - sj] + phi[u + sj]) + idz2 * (phi[u - sk] + phi[u + sk]);
         R2 = R * R;
+        if (S.copy) S.copy[B.node0 + u] = phi[u];
     }
     double t = block_sum(R2, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -806,6 +904,12 @@ struct pic_ensemble {
     int *sw_blk_box = nullptr, *rs_blk_box = nullptr;
     long *sw_blk0 = nullptr, *rs_blk0 = nullptr, *rs_blkn = nullptr;
     long n_sw_blk = 0, n_rs_blk = 0;
+    // the same boxes in blocks of ENS_SB^3 interior nodes (k_ens_sor_block) and phi's second
+    // buffer; sor_blocks false: one launch of k_ens_sweep per half-sweep (PIC_ENS_SOR_BLOCKS=0)
+    int4 *sblk = nullptr;
+    long n_sblk = 0;
+    double *phi2 = nullptr;
+    bool sor_blocks = true;
     double *rs_part = nullptr;
     int *box_it = nullptr, *done = nullptr, *ndone = nullptr, *ndone_host = nullptr;
     hipGraph_t graph_c = nullptr, graph_e = nullptr;
@@ -947,9 +1051,23 @@ EnsSweepArgs sweep_args(pic_ensemble *e, bool resid) {
 int launch_sor_chunk(pic_ensemble *e, int its, double tol) {
     if (!e->n_large) return PIC_OK;
     EnsSweepArgs SW = sweep_args(e, false), RS = sweep_args(e, true);
-    for (int it = 0; it < its; it++)
-        for (int color = 0; color < 2; color++)
-            k_ens_sweep<<<(unsigned)e->n_sw_blk, 256, 0, e->stream>>>(SW, color);
+    if (e->sor_blocks) {
+        // one launch per iteration, alternating the phi buffers; the residual pass reads the
+        // last one and, after an odd count, copies it back into phi
+        EnsBlockArgs A{e->box, e->sblk, e->phi, e->phi2, e->rho, e->done, 1.4};  // ch2/PotentialSolver.cpp:40
+        for (int it = 0; it < its; it++) {
+            k_ens_sor_block<ENS_T><<<(unsigned)e->n_sblk, 256, sor_block_lds(ENS_T), e->stream>>>(A);
+            double *next = const_cast<double *>(A.src);  // both are phi buffers of this ensemble
+            A.src = A.dst;
+            A.dst = next;
+        }
+        RS.phi = const_cast<double *>(A.src);
+        RS.copy = (its & 1) ? e->phi : nullptr;
+    } else {
+        for (int it = 0; it < its; it++)
+            for (int color = 0; color < 2; color++)
+                k_ens_sweep<<<(unsigned)e->n_sw_blk, 256, 0, e->stream>>>(SW, color);
+    }
     k_ens_resid<<<(unsigned)e->n_rs_blk, 256, 0, e->stream>>>(RS, e->rs_part);
     EnsCheckArgs C{e->box, e->sor_large, e->rs_blk0, e->rs_blkn, e->rs_part, tol,
                    e->box_it, e->done, e->iters, e->ndone, e->res};
@@ -1241,6 +1359,18 @@ int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *m
             PIC_HIP(hipMemcpy(*lp[k], lv[k]->data(), sizeof(long) * nbox, hipMemcpyHostToDevice));
         }
         if ((rc = ens_alloc((void **)&e->rs_part, sizeof(double) * std::max<long>(e->n_rs_blk, 1)))) return fail(rc);
+        std::vector<int4> sb;
+        for (int b : large) {
+            const EnsBox &B = e->hbox[b];
+            for (int z = 1; z < B.nk - 1; z += ENS_SB)
+                for (int y = 1; y < B.nj - 1; y += ENS_SB)
+                    for (int x = 1; x < B.ni - 1; x += ENS_SB) sb.push_back(make_int4(b, x, y, z));
+        }
+        e->n_sblk = (long)sb.size();
+        if ((rc = ens_alloc((void **)&e->sblk, sizeof(int4) * std::max<size_t>(sb.size(), 1)))) return fail(rc);
+        if (!sb.empty()) PIC_HIP(hipMemcpy(e->sblk, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
+        if (const char *t = getenv("PIC_ENS_SOR_BLOCKS")) e->sor_blocks = atoi(t) != 0;
+        if (sor_block_lds(ENS_T) > lds_cap) e->sor_blocks = false;
         int **fl[3] = {&e->box_it, &e->done, &e->ndone};
         for (int k = 0; k < 3; k++) {
             if ((rc = ens_alloc((void **)fl[k], sizeof(int) * (size_t)(nbox + 1)))) return fail(rc);
@@ -1269,6 +1399,9 @@ int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *m
     PIC_HIP(hipMemset(e->den, 0, sizeof(double) * tn * nsp));
     PIC_HIP(hipMemset(e->rho, 0, sizeof(double) * tn));
     PIC_HIP(hipMemset(e->phi, 0, sizeof(double) * tn));
+    // phi's second buffer: the walls are 0 in both and no kernel writes them
+    if ((rc = ens_alloc((void **)&e->phi2, sizeof(double) * tn))) return fail(rc);
+    PIC_HIP(hipMemset(e->phi2, 0, sizeof(double) * tn));
     PIC_HIP(hipMemset(e->ef32, 0, sizeof(float4) * tn));
     if (e->ef64) PIC_HIP(hipMemset(e->ef64, 0, sizeof(double4) * tn));
     // particles: two buffers (the second is the sort's target)
@@ -1321,7 +1454,7 @@ int pic_ensemble_destroy(pic_ensemble *e) {
     void *ptrs[] = {e->box, e->rng, e->chunk, e->blk_box, e->blk0, e->pmem[0], e->pmem[1], e->raw, e->den, e->rho,
                     e->phi, e->charges_dev, e->ef32, e->ef64, e->iters, e->res, e->sor_small, e->sor_large, e->cnt,
                     e->off, e->scan_tmp, e->diag_part, e->sw_blk_box, e->rs_blk_box, e->sw_blk0, e->rs_blk0,
-                    e->rs_blkn, e->rs_part, e->box_it, e->done, e->ndone};
+                    e->rs_blkn, e->rs_part, e->box_it, e->done, e->ndone, e->sblk, e->phi2};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (e->ndone_host) hipHostFree(e->ndone_host);
