@@ -548,25 +548,40 @@ __global__ void __launch_bounds__(256) k_ens_sor_block(EnsBlockArgs S) {
         if (n < NW) sp[n] = v[t];
     }
     __syncthreads();
+    // one node of a half-sweep: the q-th of colour `color` in the region [ia, ia + mi) x
+    // [ja, ja + mj) x [ka, ...), its rows colour-strided (half = (mi + 1) / 2 slots per row)
+    auto update = [&](int q, int color, int ia, int ja, int ka, int mi, int mj, int half) {
+        const int r = q / half, s = q - r * half;
+        const int gj = ja + r % mj, gk = ka + r / mj;
+        const int gi = ia + 2 * s + (((ia + gj + gk) & 1) != color ? 1 : 0);
+        if (gi >= ia + mi) return;
+        const int l = (gi - i0) + W * ((gj - j0) + W * (gk - k0));
+        const double phi_new = (rho[gi + sj * gj + sk * gk] / EPS_0 + idx2 * (sp[l - 1] + sp[l + 1]) +
+                                idy2 * (sp[l - W] + sp[l + W]) + idz2 * (sp[l - W * W] + sp[l + W * W])) /
+                               denom;
+        sp[l] = sp[l] + S.w * (phi_new - sp[l]);
+    };
+    // a block whose grown regions all lie inside the box's interior: sizes known at compile
+    // time (the index divisions become multiplications)
+    const bool full = bk.y - (T - 1) >= 1 && bk.y + ENS_SB + (T - 1) <= B.ni - 1 && bk.z - (T - 1) >= 1 &&
+                      bk.z + ENS_SB + (T - 1) <= B.nj - 1 && bk.w - (T - 1) >= 1 &&
+                      bk.w + ENS_SB + (T - 1) <= B.nk - 1;
+#pragma unroll
     for (int h = 0; h < T; h++) {
         const int color = h & 1, e = T - 1 - h;
-        // interior nodes of the block grown by e: [a, b) per axis
-        const int ia = max(bk.y - e, 1), ib = min(bk.y + ENS_SB + e, B.ni - 1);
-        const int ja = max(bk.z - e, 1), jb = min(bk.z + ENS_SB + e, B.nj - 1);
-        const int ka = max(bk.w - e, 1), kb = min(bk.w + ENS_SB + e, B.nk - 1);
-        const int mi = ib - ia, mj = jb - ja, mk = kb - ka;
-        const int half = (mi + 1) / 2;
-        const int cnt = (mi > 0 && mj > 0 && mk > 0) ? half * mj * mk : 0;
-        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-            const int r = q / half, s = q - r * half;
-            const int gj = ja + r % mj, gk = ka + r / mj;
-            const int gi = ia + 2 * s + (((ia + gj + gk) & 1) != color ? 1 : 0);
-            if (gi >= ib) continue;
-            const int l = (gi - i0) + W * ((gj - j0) + W * (gk - k0));
-            const double phi_new = (rho[gi + sj * gj + sk * gk] / EPS_0 + idx2 * (sp[l - 1] + sp[l + 1]) +
-                                    idy2 * (sp[l - W] + sp[l + W]) + idz2 * (sp[l - W * W] + sp[l + W * W])) /
-                                   denom;
-            sp[l] = sp[l] + S.w * (phi_new - sp[l]);
+        if (full) {
+            const int m = ENS_SB + 2 * e, half = (m + 1) / 2;
+            for (int q = threadIdx.x; q < half * m * m; q += blockDim.x)
+                update(q, color, bk.y - e, bk.z - e, bk.w - e, m, m, half);
+        } else {
+            // interior nodes of the block grown by e: [a, b) per axis
+            const int ia = max(bk.y - e, 1), ib = min(bk.y + ENS_SB + e, B.ni - 1);
+            const int ja = max(bk.z - e, 1), jb = min(bk.z + ENS_SB + e, B.nj - 1);
+            const int ka = max(bk.w - e, 1), kb = min(bk.w + ENS_SB + e, B.nk - 1);
+            const int mi = ib - ia, mj = jb - ja, mk = kb - ka;
+            const int half = (mi + 1) / 2;
+            const int cnt = (mi > 0 && mj > 0 && mk > 0) ? half * mj * mk : 0;
+            for (int q = threadIdx.x; q < cnt; q += blockDim.x) update(q, color, ia, ja, ka, mi, mj, half);
         }
         __syncthreads();
     }
