@@ -149,25 +149,63 @@ __global__ void __launch_bounds__(256) k_ens_push(EnsPushArgs A) {
     const int wn = s_wn;
     for (int q = threadIdx.x; q < wn; q += blockDim.x) sW[q] = 0.0;
     __syncthreads();
-    auto deposit = [&](long b, double d0, double d1, double d2, double w) {
+    // CIC deposit of a lane's particle at base node b (act: the lane has one).  The
+    // particles are sorted by cell, so in a dense box most lanes of a wavefront share a few
+    // cells and their atomics would queue on the same 8 addresses.  The 8 weights are first
+    // summed along each run of adjacent lanes with the same b (shuffles bounded by the run's
+    // last lane; between two sorts a cell can recur further on, in a run of its own), and the
+    // run's first lane deposits the sums.
+    const int lane = threadIdx.x & 63;
+    auto deposit = [&](bool act, long b, double d0, double d1, double d2, double w) {
+        double v[8];
+#pragma unroll
         for (int n = 0; n < 8; n++) {
             int cb = ens_corner(n);
             double wi = (cb & 1) ? d0 : 1 - d0;
             double wj = (cb & 2) ? d1 : 1 - d1;
             double wk = (cb & 4) ? d2 : 1 - d2;
+            v[n] = w * wi * wj * wk;
+        }
+        const int key = act ? (int)b : -1 - lane;  // idle lanes: runs of their own
+        const int kprev = __shfl_up(key, 1);
+        const bool head = lane == 0 || kprev != key;
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long above = heads & ~((2ULL << lane) - 1ULL);
+        const int end = above ? __ffsll((long long)above) - 2 : 63;  // the run's last lane
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const bool same = lane + off <= end;
+#pragma unroll
+            for (int n = 0; n < 8; n++) {
+                const double y = __shfl_down(v[n], off);
+                if (same) v[n] += y;
+            }
+        }
+        if (!act || !head) return;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            int cb = ens_corner(n);
             long u = b + ((cb & 1) ? 1 : 0) + ((cb & 2) ? sj : 0) + ((cb & 4) ? sk : 0);
-            double v = w * wi * wj * wk;
             long q = u - w0;
             if (q >= 0 && q < wn)
-                atomicAdd(&sW[q], v);
+                atomicAdd(&sW[q], v[n]);
             else
-                atomicAdd(&raw[B.node0 + u], v);
+                atomicAdd(&raw[B.node0 + u], v[n]);
         }
     };
-    for (uint32_t t = threadIdx.x; t < ch.count; t += blockDim.x) {
-        long p = R.p0 + ch.first + t;
+    // a trip count the same for every lane, so that the whole wavefront takes part in the
+    // deposit's shuffles
+    const uint32_t trips = (ch.count + blockDim.x - 1) / blockDim.x;
+    for (uint32_t it = 0; it < trips; it++) {
+        const uint32_t t = threadIdx.x + it * blockDim.x;
+        const bool act = t < ch.count;
+        long p = R.p0 + ch.first + (act ? t : 0);
         if constexpr (PREC == PIC_MIXED) {
-            float4 a = A.P.rec[2 * p], v = A.P.rec[2 * p + 1];
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), v = a;
+            if (act) {
+                a = A.P.rec[2 * p];
+                v = A.P.rec[2 * p + 1];
+            }
             int c = __float_as_int(a.x);
             int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
             float o[3] = {a.y, a.z, a.w};
@@ -227,9 +265,11 @@ __global__ void __launch_bounds__(256) k_ens_push(EnsPushArgs A) {
                 cc[d] = c2;
                 o[d] = o2;
             }
-            A.P.rec[2 * p] = make_float4(__int_as_float(pack_cell(cc[0], cc[1], cc[2])), o[0], o[1], o[2]);
-            A.P.rec[2 * p + 1] = make_float4(vel[0], vel[1], vel[2], v.w);
-            deposit(cc[0] + sj * cc[1] + sk * cc[2], o[0], o[1], o[2], v.w);
+            if (act) {
+                A.P.rec[2 * p] = make_float4(__int_as_float(pack_cell(cc[0], cc[1], cc[2])), o[0], o[1], o[2]);
+                A.P.rec[2 * p + 1] = make_float4(vel[0], vel[1], vel[2], v.w);
+            }
+            deposit(act, cc[0] + sj * cc[1] + sk * cc[2], o[0], o[1], o[2], v.w);
         } else {
             double X[3] = {A.P.x[p], A.P.y[p], A.P.z[p]};
             double V[3] = {A.P.vx[p], A.P.vy[p], A.P.vz[p]};
@@ -277,14 +317,16 @@ __global__ void __launch_bounds__(256) k_ens_push(EnsPushArgs A) {
                     V[a] *= -1.0;
                 }
             }
-            A.P.x[p] = X[0];
-            A.P.y[p] = X[1];
-            A.P.z[p] = X[2];
-            A.P.vx[p] = V[0];
-            A.P.vy[p] = V[1];
-            A.P.vz[p] = V[2];
+            if (act) {
+                A.P.x[p] = X[0];
+                A.P.y[p] = X[1];
+                A.P.z[p] = X[2];
+                A.P.vx[p] = V[0];
+                A.P.vy[p] = V[1];
+                A.P.vz[p] = V[2];
+            }
             cell(X, c, d);
-            deposit(c[0] + sj * c[1] + sk * c[2], d[0], d[1], d[2], A.P.w[p]);
+            deposit(act, c[0] + sj * c[1] + sk * c[2], d[0], d[1], d[2], A.P.w[p]);
         }
     }
     __syncthreads();
