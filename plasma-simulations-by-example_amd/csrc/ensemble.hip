@@ -11,12 +11,12 @@
 //     atomic per non-zero node; deposits that fall outside the window go straight to global
 //     atomics;
 //   - k_ens_finish: den = raw / node_vol per species (clearing raw), rho = sum q_s den_s;
-//   - k_ens_sor: one workgroup per box whose phi fits in LDS runs that box's whole SOR solve
-//     (sweeps, residual, stopping test); the larger boxes are swept by one launch per
-//     iteration over all of them (k_ens_sor_block: blocks of 8^3 nodes, both half-sweeps
-//     from an LDS window, into a second phi buffer; or k_ens_sweep, one launch per
-//     half-sweep), chunks of 25 iterations and a per-box residual check (k_ens_resid,
-//     k_ens_check), converged boxes skipping at once;
+//   - SOR: one launch per iteration over all boxes (k_ens_sor_block: blocks of 8^3 nodes,
+//     both half-sweeps from an LDS window, into a second phi buffer), chunks of 25
+//     iterations and a per-box residual check (k_ens_resid, k_ens_check), converged boxes
+//     skipping at once.  When every box's phi fits in LDS, one workgroup per box runs its
+//     whole solve there instead (k_ens_sor); with PIC_ENS_SOR_BLOCKS=0 those boxes do so and
+//     the others get one launch per half-sweep (k_ens_sweep);
 //   - k_ens_ef: computeEF of every box.
 // The step is captured in three hipGraphs (push + density + the first SOR iteration, a chunk
 // of 25 SOR iterations, computeEF) and replayed; the host reads the count of converged boxes
@@ -1286,6 +1286,12 @@ int box_range_check(pic_ensemble *e, int b, int s) {
 
 extern "C" {
 
+// the blocked SOR of the boxes outside LDS (default), or one launch per half-sweep
+static bool sor_blocks_env() {
+    const char *t = getenv("PIC_ENS_SOR_BLOCKS");
+    return !(t && atoi(t) == 0);
+}
+
 int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *mass, const double *charge,
                         const long *np, int precision, int track_ids, pic_ensemble **out) {
     PIC_CHECK(out && boxes && mass && charge && np && nbox > 0 && nsp > 0, PIC_ERR_ARG, "null or empty argument");
@@ -1362,6 +1368,15 @@ int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *m
     e->total_nodes = node0;
     e->total_cells = cell0;
     e->np = p0;
+    // With boxes outside LDS, one launch per iteration runs anyway until the largest has
+    // converged, and it is latency-bound: the boxes that fit in LDS join those launches
+    // rather than queueing a single-workgroup solve of their own before them.
+    if (!large.empty() && !small.empty() && sor_blocks_env()) {
+        large.insert(large.end(), small.begin(), small.end());
+        std::sort(large.begin(), large.end());
+        small.clear();
+        max_small = 0;
+    }
     e->n_small = (int)small.size();
     e->n_large = (int)large.size();
     e->sor_lds = max_small;
@@ -1426,7 +1441,7 @@ int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *m
         e->n_sblk = (long)sb.size();
         if ((rc = ens_alloc((void **)&e->sblk, sizeof(int4) * std::max<size_t>(sb.size(), 1)))) return fail(rc);
         if (!sb.empty()) PIC_HIP(hipMemcpy(e->sblk, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
-        if (const char *t = getenv("PIC_ENS_SOR_BLOCKS")) e->sor_blocks = atoi(t) != 0;
+        e->sor_blocks = sor_blocks_env();
         if (sor_block_lds(ENS_T) > lds_cap) e->sor_blocks = false;
         int **fl[3] = {&e->box_it, &e->done, &e->ndone};
         for (int k = 0; k < 3; k++) {

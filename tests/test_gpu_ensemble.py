@@ -10,7 +10,9 @@ own eager work), and each replay is checked against the oracle run on the same i
   - density and rho against Species::computeNumberDensity / World::computeChargeDensity of
     the oracle on the GPU's particles;
   - E bit-identical to computeEF of the GPU's phi; phi passes the reference's stopping test.
-Box sizes span both SOR paths (phi in LDS, phi in global memory).
+Box sizes span every SOR path: boxes whose phi fits in LDS (9^3 to 25x19x29) and boxes whose
+phi does not (33^3); by default all of them then run in the blocked launches, and with
+PIC_ENS_SOR_BLOCKS=0 the first kind is solved in LDS and the second by half-sweep launches.
 """
 import numpy as np
 import pytest
@@ -61,8 +63,14 @@ def by_id(parts):
     return pos[o], vel[o], mpw[o]
 
 
+@pytest.mark.parametrize("sor", ["blocks", "half_sweeps"])
 @pytest.mark.parametrize("prec", [P.FP64, P.MIXED])
-def test_ensemble_steps_match_oracle_box_by_box(prec):
+def test_ensemble_steps_match_oracle_box_by_box(prec, sor, monkeypatch):
+    """sor: every box in the blocked SOR launches (the default when some box's phi does not
+    fit in LDS), or the boxes that fit solved in LDS by one workgroup each and the others by
+    one launch per half-sweep (PIC_ENS_SOR_BLOCKS=0)."""
+    if sor == "half_sweeps":
+        monkeypatch.setenv("PIC_ENS_SOR_BLOCKS", "0")
     ens, grids, loads = build(prec)
     ens.solve_initial(20000, TOL)
     it, res = ens.solver_status()
@@ -135,12 +143,12 @@ def test_graph_replays_match_eager_steps():
 
 
 def test_blocked_sor_matches_half_sweep_launches(monkeypatch):
-    """The SOR of the boxes outside LDS in blocks of 8^3 nodes, one launch per iteration from
-    an LDS window into phi's second buffer (the default), against one launch per half-sweep
-    over the whole boxes (PIC_ENS_SOR_BLOCKS=0), on identical inputs: the same iteration
-    count for every box at every solve, and phi alike to the deposit's atomic-order
-    differences.  The 33^3 and 25x19x29 boxes are swept this way, the second with partial
-    blocks at its upper faces."""
+    """The blocked SOR (the default: every box in blocks of 8^3 nodes, one launch per
+    iteration from an LDS window into phi's second buffer) against PIC_ENS_SOR_BLOCKS=0 (the
+    boxes whose phi fits in LDS solved there by one workgroup each, the 33^3 box by one launch
+    per half-sweep), on identical inputs: the same iteration count for every box at every
+    solve, and phi alike to the deposit's atomic-order differences.  Most boxes end in partial
+    blocks (interiors of 7 to 31 nodes)."""
     runs = []
     for mode in ("1", "0"):
         monkeypatch.setenv("PIC_ENS_SOR_BLOCKS", mode)
