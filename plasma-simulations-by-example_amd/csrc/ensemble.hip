@@ -11,8 +11,8 @@
 //     atomic per non-zero node; deposits that fall outside the window go straight to global
 //     atomics;
 //   - k_ens_finish: den = raw / node_vol per species (clearing raw), rho = sum q_s den_s;
-//   - SOR: one launch per iteration over all boxes (k_ens_sor_block: blocks of 8^3 nodes,
-//     both half-sweeps from an LDS window, into a second phi buffer), chunks of 25
+//   - SOR: one launch per two iterations over all boxes (k_ens_sor_block: blocks of 8^3
+//     nodes, four half-sweeps from an LDS window, into a second phi buffer), chunks of 25
 //     iterations and a per-box residual check (k_ens_resid, k_ens_check), converged boxes
 //     skipping at once.  When every box's phi fits in LDS, one workgroup per box runs its
 //     whole solve there instead (k_ens_sor); with PIC_ENS_SOR_BLOCKS=0 those boxes do so and
@@ -534,18 +534,18 @@ __global__ void __launch_bounds__(256) k_ens_sweep(EnsSweepArgs S, int color) {
     phi[u] = phi[u] + S.w * (phi_new - phi[u]);
 }
 
-// ---- the same SOR, one iteration per launch ---------------------------------------------------
+// ---- the same SOR, two iterations per launch --------------------------------------------------
 // A launch of k_ens_sweep costs about 13 us however few nodes it updates, and the largest box
 // needs about 1400 iterations, most of them after the smaller boxes have converged.  Here a
 // workgroup takes a block of ENS_SB^3 interior nodes of one box and its window with a T-node
 // rim in LDS, runs T half-sweeps there (red first), each on the nodes the later ones still
 // need (the block grown by T - 1 - h nodes), and writes the block's nodes to the other phi
 // buffer.  Every node gets the same updates, from the same values, as from T launches of
-// k_ens_sweep; the rim's nodes are recomputed by each block that needs them.  T = 2 (one
-// iteration) with 8^3 blocks measured best: larger blocks leave the largest box too few
-// workgroups, and T = 4 loads a window 2.4 times the block's nodes.
+// k_ens_sweep; the rim's nodes are recomputed by each block that needs them.  T = 4 (two
+// iterations) with 8^3 blocks measured best: larger blocks leave the largest box too few
+// workgroups, and a wider rim loads windows many times the block's nodes.
 constexpr int ENS_SB = 8;
-constexpr int ENS_T = 2;
+constexpr int ENS_T = 4;
 
 struct EnsBlockArgs {
     const EnsBox *box;
@@ -1109,17 +1109,26 @@ int launch_sor_chunk(pic_ensemble *e, int its, double tol) {
     if (!e->n_large) return PIC_OK;
     EnsSweepArgs SW = sweep_args(e, false), RS = sweep_args(e, true);
     if (e->sor_blocks) {
-        // one launch per iteration, alternating the phi buffers; the residual pass reads the
-        // last one and, after an odd count, copies it back into phi
+        // one launch per two iterations (and one for an odd last), alternating the phi
+        // buffers; the residual pass reads the last one and, after an odd count of launches,
+        // copies it back into phi
         EnsBlockArgs A{e->box, e->sblk, e->phi, e->phi2, e->rho, e->done, 1.4};  // ch2/PotentialSolver.cpp:40
-        for (int it = 0; it < its; it++) {
-            k_ens_sor_block<ENS_T><<<(unsigned)e->n_sblk, 256, sor_block_lds(ENS_T), e->stream>>>(A);
+        int launches = 0;
+        for (int it = 0; it < its;) {
+            if (its - it >= 2) {
+                k_ens_sor_block<ENS_T><<<(unsigned)e->n_sblk, 256, sor_block_lds(ENS_T), e->stream>>>(A);
+                it += ENS_T / 2;
+            } else {  // an odd last iteration
+                k_ens_sor_block<2><<<(unsigned)e->n_sblk, 256, sor_block_lds(2), e->stream>>>(A);
+                it += 1;
+            }
+            launches++;
             double *next = const_cast<double *>(A.src);  // both are phi buffers of this ensemble
             A.src = A.dst;
             A.dst = next;
         }
         RS.phi = const_cast<double *>(A.src);
-        RS.copy = (its & 1) ? e->phi : nullptr;
+        RS.copy = (launches & 1) ? e->phi : nullptr;
     } else {
         for (int it = 0; it < its; it++)
             for (int color = 0; color < 2; color++)
@@ -1368,8 +1377,8 @@ int pic_ensemble_create(int nbox, const pic_box *boxes, int nsp, const double *m
     e->total_nodes = node0;
     e->total_cells = cell0;
     e->np = p0;
-    // With boxes outside LDS, one launch per iteration runs anyway until the largest has
-    // converged, and it is latency-bound: the boxes that fit in LDS join those launches
+    // With boxes outside LDS, the blocked launches run anyway until the largest has
+    // converged, and they are latency-bound: the boxes that fit in LDS join those launches
     // rather than queueing a single-workgroup solve of their own before them.
     if (!large.empty() && !small.empty() && sor_blocks_env()) {
         large.insert(large.end(), small.begin(), small.end());

@@ -143,8 +143,8 @@ def test_graph_replays_match_eager_steps():
 
 
 def test_blocked_sor_matches_half_sweep_launches(monkeypatch):
-    """The blocked SOR (the default: every box in blocks of 8^3 nodes, one launch per
-    iteration from an LDS window into phi's second buffer) against PIC_ENS_SOR_BLOCKS=0 (the
+    """The blocked SOR (the default: every box in blocks of 8^3 nodes, one launch per two
+    iterations from an LDS window into phi's second buffer) against PIC_ENS_SOR_BLOCKS=0 (the
     boxes whose phi fits in LDS solved there by one workgroup each, the 33^3 box by one launch
     per half-sweep), on identical inputs: the same iteration count for every box at every
     solve, and phi alike to the deposit's atomic-order differences.  Most boxes end in partial
