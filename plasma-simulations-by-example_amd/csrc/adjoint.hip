@@ -32,6 +32,8 @@
 #include <cmath>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "picamd_internal.h"
 #include "stencil.h"
 
@@ -145,6 +147,31 @@ __device__ inline void scatter_atomic(const Grid &g, double *f, const Cic &q, do
         double wk = (cb & 4) ? q.d[2] : 1 - q.d[2];
         atomicAdd(&f[q.b + corner_off(g, n)], v * wi * wj * wk);
     }
+}
+
+// Sums of NV values per lane along runs of adjacent lanes with the same key (every lane of
+// the wavefront calls it; a lane with nothing to add passes a key of its own): the reverse
+// push visits the particles in cell order, so a wavefront's lanes share a few cells, and each
+// run's first lane then issues the run's atomics in place of one set per lane.  Returns
+// whether this lane heads its run.
+template <int NV>
+__device__ inline bool run_sums(int key, double (&v)[NV]) {
+    const int lane = (int)__lane_id();
+    const int kprev = __shfl_up(key, 1);
+    const bool head = lane == 0 || kprev != key;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long above = heads & ~((2ULL << lane) - 1ULL);
+    const int end = above ? __ffsll((long long)above) - 2 : 63;  // the run's last lane
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const bool same = lane + off <= end;
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+            const double y = __shfl_down(v[n], off);
+            if (same) v[n] += y;
+        }
+    }
+    return head;
 }
 
 // ---- soft absorption ----------------------------------------------------------------------
@@ -323,6 +350,14 @@ __global__ void __launch_bounds__(256) k_dsph_deposit(Grid g, DState st, long n,
     if (w > 0.0) scatter_atomic(g, den_raw, cic_of(g, st.x[p], st.y[p], st.z[p]), w);
 }
 
+// sort keys of slots [0, n): the base node of the particle's cell (live ones) or nn (others)
+__global__ void __launch_bounds__(256) k_dsph_keys(Grid g, DState st, long n, unsigned *keys, int *vals) {
+    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    keys[p] = st.w[p] > 0.0 ? (unsigned)cic_of(g, st.x[p], st.y[p], st.z[p]).b : (unsigned)g.nn;
+    vals[p] = (int)p;
+}
+
 // den = den_raw / node_vol (Species::computeNumberDensity), rho = q den
 // (World::computeChargeDensity); den_out (nullable) keeps den
 __global__ void __launch_bounds__(256)
@@ -354,6 +389,8 @@ struct AdjPush {
     DParams P;
     long n;       // slots live before the push (incl. this step's injection)
     long n_old;   // slots that existed before this step's injection
+    const int *perm;  // slots [0, nperm) in cell order
+    long nperm;
     double sbar;  // dL/d(weight taken by the sphere)
     double winj_per_vd;  // d(injected weight)/d(v_drift)
 };
@@ -369,9 +406,17 @@ __global__ void __launch_bounds__(256)
     __shared__ double sh[4];
     const DParams &P = A.P;
     const Grid &g = P.g;
-    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    // thread t takes slot perm[t] for t < nperm (the segment's first slots in cell order) and
+    // slot t after them (injected later in the segment)
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long p = t < A.nperm ? (long)A.perm[t] : t;
     double gvd = 0.0;
-    if (p < A.n) {
+    bool dep = false;  // this lane adds eb3 (corner by corner) into Ebar at base node cb
+    long cb0 = 0;
+    double eb3[24];
+#pragma unroll
+    for (int q = 0; q < 24; q++) eb3[q] = 0.0;
+    if (t < A.n) {
         double w = st.w[p];
         if (w > 0.0) {
             double x = st.x[p], y = st.y[p], z = st.z[p];
@@ -424,10 +469,12 @@ __global__ void __launch_bounds__(256)
                     e1 -= vb[1] * r;
                     e2 -= vb[2] * r;
                 }
-                atomicAdd(&ebx[u], e0);
-                atomicAdd(&eby[u], e1);
-                atomicAdd(&ebz[u], e2);
+                eb3[3 * n] = e0;
+                eb3[3 * n + 1] = e1;
+                eb3[3 * n + 2] = e2;
             }
+            dep = true;
+            cb0 = c.b;
             xb[0] += gx[0] / g.dh[0];
             xb[1] += gx[1] / g.dh[1];
             xb[2] += gx[2] / g.dh[2];
@@ -449,6 +496,17 @@ __global__ void __launch_bounds__(256)
             adj.x[p] = adj.y[p] = adj.z[p] = 0.0;
             adj.vx[p] = adj.vy[p] = adj.vz[p] = 0.0;
             adj.w[p] = 0.0;
+        }
+    }
+    // E's transpose into the nodes, summed along runs of lanes with the same cell first
+    const bool head = run_sums<24>(dep ? (int)cb0 : -1 - (int)__lane_id(), eb3);
+    if (dep && head) {
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            long u = cb0 + corner_off(g, n);
+            atomicAdd(&ebx[u], eb3[3 * n]);
+            atomicAdd(&eby[u], eb3[3 * n + 1]);
+            atomicAdd(&ebz[u], eb3[3 * n + 2]);
         }
     }
     double s = block_sum(gvd, sh);
@@ -619,6 +677,15 @@ struct pic_adjoint {
     bool have_forward = false;
     double w_init = 0, w_inj_per_vd = 0, vth = 0;
     double stats[4] = {0, 0, 0, 0};  // forward ms, backward ms, Newton solves, adjoint CG its
+    // the slots of a segment's first state in cell order (stable radix sort by cell), the
+    // order in which the reverse pushes of the segment visit them
+    int *perm = nullptr;
+    long nperm = 0;
+    unsigned *keys = nullptr, *keys2 = nullptr;
+    int *vals = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int key_bits = 32;
 };
 
 namespace {
@@ -716,6 +783,19 @@ int load(pic_adjoint *a, const DState &st, long n, long slot0, int step, int inj
     L.w = inject ? a->w_inj_per_vd * a->theta[0] : a->w_init;
     k_dsph_load<<<nbl(n), 256, 0, a->w->stream>>>(L, a->w->ef64, st);
     PIC_LAUNCH_CHECK();
+    return PIC_OK;
+}
+
+// the cell order of slots [0, n) of st, for the pushes until the next call
+int order_by_cell(pic_adjoint *a, const DState &st, long n) {
+    pic_world *w = a->w;
+    a->nperm = n;
+    if (n <= 0) return PIC_OK;
+    k_dsph_keys<<<nbl(n), 256, 0, w->stream>>>(a->g, st, n, a->keys, a->vals);
+    PIC_LAUNCH_CHECK();
+    size_t bytes = a->sort_tmp_bytes;
+    PIC_HIP(hipcub::DeviceRadixSort::SortPairs(a->sort_tmp, bytes, a->keys, a->keys2, a->vals, a->perm, (int)n, 0,
+                                               a->key_bits, w->stream));
     return PIC_OK;
 }
 
@@ -906,6 +986,35 @@ int pic_adjoint_create(const pic_adjoint_config *cfg, pic_adjoint **out) {
     if ((rc = alloc_d(a, &a->phi_ck, (size_t)nn * a->nck))) return fail(rc);
     if ((rc = alloc_d(a, &a->acc, 8))) return fail(rc);
     if ((rc = grow_part(a, nbl(std::max(a->cap, nn))))) return fail(rc);
+    {
+        // the cell-order sort of up to cap slots: keys below nn + 1
+        PIC_CHECK(a->cap < (1L << 31) && nn < (1L << 31), PIC_ERR_ARG, "too many slots or nodes for the cell sort");
+        void *m[4] = {nullptr, nullptr, nullptr, nullptr};
+        const size_t capn = (size_t)std::max<long>(a->cap, 1);
+        size_t sz[3] = {sizeof(int) * capn, sizeof(unsigned) * capn, sizeof(unsigned) * capn};
+        for (int i = 0; i < 3; i++) {
+            hipError_t he = hipMalloc(&m[i], sz[i]);
+            if (he != hipSuccess) return fail(hip_fail(he, "hipMalloc (cell sort)", __FILE__, __LINE__));
+            a->mem.push_back(m[i]);
+        }
+        a->perm = (int *)m[0];
+        a->keys = (unsigned *)m[1];
+        a->keys2 = (unsigned *)m[2];
+        hipError_t he = hipMalloc(&m[3], sizeof(int) * capn);
+        if (he != hipSuccess) return fail(hip_fail(he, "hipMalloc (cell sort)", __FILE__, __LINE__));
+        a->mem.push_back(m[3]);
+        a->vals = (int *)m[3];
+        a->key_bits = 1;
+        while ((1L << a->key_bits) <= nn) a->key_bits++;
+        size_t bytes = 0;
+        he = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, a->keys, a->keys2, a->vals, a->perm, (int)capn, 0,
+                                                a->key_bits, a->w->stream);
+        if (he != hipSuccess) return fail(hip_fail(he, "hipcub radix sort (size)", __FILE__, __LINE__));
+        he = hipMalloc(&a->sort_tmp, std::max<size_t>(bytes, 1));
+        if (he != hipSuccess) return fail(hip_fail(he, "hipMalloc (cell sort)", __FILE__, __LINE__));
+        a->mem.push_back(a->sort_tmp);
+        a->sort_tmp_bytes = bytes;
+    }
     hipError_t e = hipHostMalloc((void **)&a->acc_host, sizeof(double) * 8, hipHostMallocDefault);
     if (e != hipSuccess) return fail(hip_fail(e, "hipHostMalloc", __FILE__, __LINE__));
     *out = a;
@@ -996,6 +1105,8 @@ int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, co
         // recompute the segment from its checkpoint, keeping each step's state and phi
         rc = copy_state(a, a->ck[ci], a->seg[0], n_before(a, n_first), w->stream);
         if (rc) return rc;
+        rc = order_by_cell(a, a->seg[0], n_before(a, n_first));
+        if (rc) return rc;
         PIC_HIP(hipMemcpyAsync(w->phi, a->phi_ck + (size_t)ci * nn, sizeof(double) * nn, hipMemcpyDeviceToDevice,
                                w->stream));
         rc = set_sphere_potential(a, a->theta[1]);
@@ -1027,6 +1138,8 @@ int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, co
             A.P = a->P;
             A.n = n_before(a, n) + c.n_inject;
             A.n_old = n_before(a, n);
+            A.perm = a->perm;
+            A.nperm = std::min(a->nperm, A.n);
             A.sbar = sbar;
             A.winj_per_vd = a->w_inj_per_vd;
             rc = grow_part(a, nbl(A.n));
