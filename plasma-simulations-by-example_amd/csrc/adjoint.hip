@@ -150,10 +150,10 @@ __device__ inline void scatter_atomic(const Grid &g, double *f, const Cic &q, do
 }
 
 // Sums of NV values per lane along runs of adjacent lanes with the same key (every lane of
-// the wavefront calls it; a lane with nothing to add passes a key of its own): the reverse
-// push visits the particles in cell order, so a wavefront's lanes share a few cells, and each
-// run's first lane then issues the run's atomics in place of one set per lane.  Returns
-// whether this lane heads its run.
+// the wavefront calls it; a lane with nothing to add passes a key of its own): the slots of a
+// segment are kept in cell order, so a wavefront's lanes share a few cells, and each run's
+// first lane then issues the run's atomics in place of one set per lane.  Returns whether
+// this lane heads its run.
 template <int NV>
 __device__ inline bool run_sums(int key, double (&v)[NV]) {
     const int lane = (int)__lane_id();
@@ -172,6 +172,24 @@ __device__ inline bool run_sums(int key, double (&v)[NV]) {
         }
     }
     return head;
+}
+
+// scatter_atomic for a wavefront: act, the lane deposits v at q
+__device__ inline void scatter_runs(const Grid &g, double *f, bool act, const Cic &q, double v) {
+#pragma clang fp contract(off)
+    double c[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        int cb = corner_bits(n);
+        double wi = (cb & 1) ? q.d[0] : 1 - q.d[0];
+        double wj = (cb & 2) ? q.d[1] : 1 - q.d[1];
+        double wk = (cb & 4) ? q.d[2] : 1 - q.d[2];
+        c[n] = v * wi * wj * wk;
+    }
+    const bool head = run_sums<8>(act ? (int)q.b : -1 - (int)__lane_id(), c);
+    if (!act || !head) return;
+#pragma unroll
+    for (int n = 0; n < 8; n++) atomicAdd(&f[q.b + corner_off(g, n)], c[n]);
 }
 
 // ---- soft absorption ----------------------------------------------------------------------
@@ -310,6 +328,9 @@ __global__ void __launch_bounds__(256)
     const Grid &g = P.g;
     long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double took = 0.0;
+    bool dep = false;  // this lane deposits wd at cd
+    Cic cd{};
+    double wd = 0.0;
     if (p < n) {
         double w = in.w[p];
         if (w > 0.0) {
@@ -328,7 +349,11 @@ __global__ void __launch_bounds__(256)
             double mw = soft_walls(P, X, gw);
             took = w * (1.0 - ms);
             w = w * ms * mw;
-            if (w > 0.0) scatter_atomic(g, den_raw, cic_of(g, x, y, z), w);
+            if (w > 0.0) {
+                dep = true;
+                cd = cic_of(g, x, y, z);
+                wd = w;
+            }
             out.x[p] = x;
             out.y[p] = y;
             out.z[p] = z;
@@ -338,6 +363,7 @@ __global__ void __launch_bounds__(256)
         }
         out.w[p] = w > 0.0 ? w : 0.0;
     }
+    scatter_runs(g, den_raw, dep, cd, wd);
     double s = block_sum(took, sh);
     if (threadIdx.x == 0) part_s[blockIdx.x] = s;
 }
@@ -356,6 +382,34 @@ __global__ void __launch_bounds__(256) k_dsph_keys(Grid g, DState st, long n, un
     if (p >= n) return;
     keys[p] = st.w[p] > 0.0 ? (unsigned)cic_of(g, st.x[p], st.y[p], st.z[p]).b : (unsigned)g.nn;
     vals[p] = (int)p;
+}
+
+// dst[t] = src[perm[t]] for t < n (a state into cell order)
+__global__ void __launch_bounds__(256) k_dsph_gather(DState src, DState dst, const int *perm, long n) {
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    long p = perm[t];
+    dst.x[t] = src.x[p];
+    dst.y[t] = src.y[p];
+    dst.z[t] = src.z[p];
+    dst.vx[t] = src.vx[p];
+    dst.vy[t] = src.vy[p];
+    dst.vz[t] = src.vz[p];
+    dst.w[t] = src.w[p];
+}
+
+// dst[perm[t]] = src[t] for t < n (adjoints back to the previous segment's slot order)
+__global__ void __launch_bounds__(256) k_dsph_scatter(DState src, DState dst, const int *perm, long n) {
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    long p = perm[t];
+    dst.x[p] = src.x[t];
+    dst.y[p] = src.y[t];
+    dst.z[p] = src.z[t];
+    dst.vx[p] = src.vx[t];
+    dst.vy[p] = src.vy[t];
+    dst.vz[p] = src.vz[t];
+    dst.w[p] = src.w[t];
 }
 
 // den = den_raw / node_vol (Species::computeNumberDensity), rho = q den
@@ -389,8 +443,6 @@ struct AdjPush {
     DParams P;
     long n;       // slots live before the push (incl. this step's injection)
     long n_old;   // slots that existed before this step's injection
-    const int *perm;  // slots [0, nperm) in cell order
-    long nperm;
     double sbar;  // dL/d(weight taken by the sphere)
     double winj_per_vd;  // d(injected weight)/d(v_drift)
 };
@@ -406,17 +458,14 @@ __global__ void __launch_bounds__(256)
     __shared__ double sh[4];
     const DParams &P = A.P;
     const Grid &g = P.g;
-    // thread t takes slot perm[t] for t < nperm (the segment's first slots in cell order) and
-    // slot t after them (injected later in the segment)
-    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const long p = t < A.nperm ? (long)A.perm[t] : t;
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double gvd = 0.0;
     bool dep = false;  // this lane adds eb3 (corner by corner) into Ebar at base node cb
     long cb0 = 0;
     double eb3[24];
 #pragma unroll
     for (int q = 0; q < 24; q++) eb3[q] = 0.0;
-    if (t < A.n) {
+    if (p < A.n) {
         double w = st.w[p];
         if (w > 0.0) {
             double x = st.x[p], y = st.y[p], z = st.z[p];
@@ -677,10 +726,10 @@ struct pic_adjoint {
     bool have_forward = false;
     double w_init = 0, w_inj_per_vd = 0, vth = 0;
     double stats[4] = {0, 0, 0, 0};  // forward ms, backward ms, Newton solves, adjoint CG its
-    // the slots of a segment's first state in cell order (stable radix sort by cell), the
-    // order in which the reverse pushes of the segment visit them
-    int *perm = nullptr;
-    long nperm = 0;
+    // Each checkpoint's state is put into cell order (stable radix sort by cell) before it is
+    // stored: perms[ci] maps its new slots to the old ones, so that the backward pass can
+    // carry the adjoints back across the checkpoint
+    int *perms = nullptr;  // nck x cap
     unsigned *keys = nullptr, *keys2 = nullptr;
     int *vals = nullptr;
     void *sort_tmp = nullptr;
@@ -786,16 +835,18 @@ int load(pic_adjoint *a, const DState &st, long n, long slot0, int step, int inj
     return PIC_OK;
 }
 
-// the cell order of slots [0, n) of st, for the pushes until the next call
-int order_by_cell(pic_adjoint *a, const DState &st, long n) {
+// slots [0, n) of src into cell order in dst (stable: slots of one cell keep their order);
+// perm receives the old slot of each new one
+int order_by_cell(pic_adjoint *a, const DState &src, const DState &dst, long n, int *perm) {
     pic_world *w = a->w;
-    a->nperm = n;
     if (n <= 0) return PIC_OK;
-    k_dsph_keys<<<nbl(n), 256, 0, w->stream>>>(a->g, st, n, a->keys, a->vals);
+    k_dsph_keys<<<nbl(n), 256, 0, w->stream>>>(a->g, src, n, a->keys, a->vals);
     PIC_LAUNCH_CHECK();
     size_t bytes = a->sort_tmp_bytes;
-    PIC_HIP(hipcub::DeviceRadixSort::SortPairs(a->sort_tmp, bytes, a->keys, a->keys2, a->vals, a->perm, (int)n, 0,
+    PIC_HIP(hipcub::DeviceRadixSort::SortPairs(a->sort_tmp, bytes, a->keys, a->keys2, a->vals, perm, (int)n, 0,
                                                a->key_bits, w->stream));
+    k_dsph_gather<<<nbl(n), 256, 0, w->stream>>>(src, dst, perm, n);
+    PIC_LAUNCH_CHECK();
     return PIC_OK;
 }
 
@@ -991,13 +1042,13 @@ int pic_adjoint_create(const pic_adjoint_config *cfg, pic_adjoint **out) {
         PIC_CHECK(a->cap < (1L << 31) && nn < (1L << 31), PIC_ERR_ARG, "too many slots or nodes for the cell sort");
         void *m[4] = {nullptr, nullptr, nullptr, nullptr};
         const size_t capn = (size_t)std::max<long>(a->cap, 1);
-        size_t sz[3] = {sizeof(int) * capn, sizeof(unsigned) * capn, sizeof(unsigned) * capn};
+        size_t sz[3] = {sizeof(int) * capn * a->nck, sizeof(unsigned) * capn, sizeof(unsigned) * capn};
         for (int i = 0; i < 3; i++) {
             hipError_t he = hipMalloc(&m[i], sz[i]);
             if (he != hipSuccess) return fail(hip_fail(he, "hipMalloc (cell sort)", __FILE__, __LINE__));
             a->mem.push_back(m[i]);
         }
-        a->perm = (int *)m[0];
+        a->perms = (int *)m[0];
         a->keys = (unsigned *)m[1];
         a->keys2 = (unsigned *)m[2];
         hipError_t he = hipMalloc(&m[3], sizeof(int) * capn);
@@ -1007,7 +1058,7 @@ int pic_adjoint_create(const pic_adjoint_config *cfg, pic_adjoint **out) {
         a->key_bits = 1;
         while ((1L << a->key_bits) <= nn) a->key_bits++;
         size_t bytes = 0;
-        he = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, a->keys, a->keys2, a->vals, a->perm, (int)capn, 0,
+        he = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, a->keys, a->keys2, a->vals, a->perms, (int)capn, 0,
                                                 a->key_bits, a->w->stream);
         if (he != hipSuccess) return fail(hip_fail(he, "hipcub radix sort (size)", __FILE__, __LINE__));
         he = hipMalloc(&a->sort_tmp, std::max<size_t>(bytes, 1));
@@ -1050,7 +1101,10 @@ int pic_adjoint_forward(pic_adjoint *a, const double theta[3], double *current, 
     for (int n = 0; n < c.steps; n++) {
         if (n % c.checkpoint_every == 0) {
             int ci = n / c.checkpoint_every;
-            rc = copy_state(a, A, a->ck[ci], n_before(a, n), w->stream);
+            // the checkpoint holds the state in cell order, and the run goes on from it
+            rc = order_by_cell(a, A, a->ck[ci], n_before(a, n), a->perms + (size_t)ci * a->cap);
+            if (rc) return rc;
+            rc = copy_state(a, a->ck[ci], A, n_before(a, n), w->stream);
             if (rc) return rc;
             PIC_HIP(hipMemcpyAsync(a->phi_ck + (size_t)ci * a->g.nn, w->phi, sizeof(double) * a->g.nn,
                                    hipMemcpyDeviceToDevice, w->stream));
@@ -1105,8 +1159,6 @@ int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, co
         // recompute the segment from its checkpoint, keeping each step's state and phi
         rc = copy_state(a, a->ck[ci], a->seg[0], n_before(a, n_first), w->stream);
         if (rc) return rc;
-        rc = order_by_cell(a, a->seg[0], n_before(a, n_first));
-        if (rc) return rc;
         PIC_HIP(hipMemcpyAsync(w->phi, a->phi_ck + (size_t)ci * nn, sizeof(double) * nn, hipMemcpyDeviceToDevice,
                                w->stream));
         rc = set_sphere_potential(a, a->theta[1]);
@@ -1138,8 +1190,6 @@ int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, co
             A.P = a->P;
             A.n = n_before(a, n) + c.n_inject;
             A.n_old = n_before(a, n);
-            A.perm = a->perm;
-            A.nperm = std::min(a->nperm, A.n);
             A.sbar = sbar;
             A.winj_per_vd = a->w_inj_per_vd;
             rc = grow_part(a, nbl(A.n));
@@ -1152,6 +1202,16 @@ int pic_adjoint_backward(pic_adjoint *a, double dcurrent, const double *dden, co
             // the solve that produced phi_n (the initial one for n = 0) at phi_n
             rc = solve_adjoint(a);
             if (rc) return rc;
+        }
+        if (ci > 0) {
+            // the adjoints of checkpoint ci's slots back to the order the previous segment ended in
+            long nb = n_before(a, n_first);
+            if (nb > 0) {
+                k_dsph_scatter<<<nbl(nb), 256, 0, w->stream>>>(adj, a->seg[0], a->perms + (size_t)ci * a->cap, nb);
+                PIC_LAUNCH_CHECK();
+                std::swap(a->adj, a->seg[0]);
+                adj = a->adj;
+            }
         }
     }
     // initial particles: v = v_drift z + thermal
