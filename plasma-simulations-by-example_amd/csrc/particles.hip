@@ -699,6 +699,7 @@ struct PushOut {
     double *diag_part;  // [NDIAG_SLOTS][gridDim.x] per-block partials, reduced on demand
     uint32_t *err;
     uint32_t *far;          // per tile: particles deposited outside the window this push
+    uint32_t *far_in;       // per tile: 1 when marked far arrivals landed in its inflow segment
     uint32_t chunk;         // particles per push workgroup (the plan's chunk size)
     uint32_t stripe;        // map entries per XCD stripe
 };
@@ -906,8 +907,6 @@ __global__ void __launch_bounds__(PUSH_THREADS)
                     deposit_lds<float, WNR>(sW, wi, wj, wk, p.ox, p.oy, p.oz, p.w);
                 else
                     deposit_lds<double, WNR>(sW, wi, wj, wk, di, dj, dk, p.w);
-            } else if (alive) {
-                deposit_global(g, O.den_extra, p);
             }
             const unsigned long long fm = __ballot(alive && wi < 0);  // far deposits, counted
             if (fm && (int)__lane_id() == __ffsll((long long)fm) - 1) atomicAdd(&sFar, (uint32_t)__popcll(fm));
@@ -930,6 +929,18 @@ __global__ void __launch_bounds__(PUSH_THREADS)
             const bool to_stage = alive && idx < 0;
             const long sidx = staging_slot(to_stage, out, O.err);
             if (to_stage) idx = sidx;
+            // A particle that deposits outside the window (always an arrival: a stayer's cell
+            // is in the tile) is marked in its slot's byte, and k_far_arrivals deposits it
+            // through its destination tile's window after the push, in place of eight global
+            // atomics per particle.
+            if (alive && wi < 0) {
+                if (idx >= 0) {
+                    out.mark[idx] = 1;
+                    if (!to_stage) O.far_in[dtile] = 1u;
+                } else {
+                    deposit_global(g, O.den_extra, p);  // dropped (staging full): deposit now
+                }
+            }
             if (idx >= 0) store_p(out, idx, p);
         }
         __syncthreads();
@@ -1020,6 +1031,75 @@ __global__ void __launch_bounds__(PUSH_THREADS)
         for (int i = 0; i < PUSH_THREADS / 64; i++) s += sRed[tid][i];
         O.diag_part[tid * gridDim.x + blockIdx.x] = s;
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// deferred far deposits of the push
+// ---------------------------------------------------------------------------------------
+// Workgroup t < ntiles: when the push flagged tile t, the marked particles of its inflow
+// segment (one byte per slot, cleared here) are deposited into an LDS window of the tile's
+// (TILE+1)^3 nodes, in the push's arithmetic; the window is then
+// added into the tile's window slot (this workgroup owns it: no atomics), or written with the
+// rest of the slot zeroed when the push left the slot unused; the flag is cleared.
+// Workgroups past ntiles walk the staging list, whose marked particles (arrivals that found
+// their segment full) deposit by global atomics.
+template <int PREC>
+__global__ void __launch_bounds__(256)
+    k_far_arrivals(Grid g, PartView out, uint32_t *far_in, float *win32, double *win64, uint8_t *win_used,
+                   double *den_extra) {
+    using PT = typename std::conditional<PREC == PIC_MIXED, PMixed, PF64>::type;
+    __shared__ double sD[TNODES];
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x;
+    if (t >= g.ntiles) {
+        const uint32_t ns = min(*out.stage_count, (uint32_t)out.cap_stage);
+        const uint32_t nth = (gridDim.x - g.ntiles) * 256u;
+        for (uint32_t q = (t - g.ntiles) * 256u + tid; q < ns; q += nth) {
+            const long idx = out.cap_bins + q;
+            if (!out.mark[idx]) continue;
+            out.mark[idx] = 0;
+            PT p;
+            load_p(out, idx, p);
+            deposit_global(g, den_extra, p);
+        }
+        return;
+    }
+    if (far_in[t] == 0u) return;
+    for (int n = tid; n < TNODES; n += 256) sD[n] = 0.0;
+    __syncthreads();
+    if (tid == 0) far_in[t] = 0u;
+    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+    const int c0x = tx * TILE, c0y = ty * TILE, c0z = tz * TILE;
+    const BinSeg bs = bin_of(out, t);
+    for (uint32_t q = tid; q < bs.c1; q += 256) {
+        const long idx = (long)bs.s1 + q;
+        if (!out.mark[idx]) continue;
+        out.mark[idx] = 0;
+        PT p;
+        load_p(out, idx, p);
+        int ci, cj, ck;
+        if constexpr (PREC == PIC_MIXED) {
+            deposit_lds<float, TN>(sD, p.ci - c0x, p.cj - c0y, p.ck - c0z, p.ox, p.oy, p.oz, p.w);
+        } else {
+            double di, dj, dk;
+            cell_of(g, p, ci, cj, ck, di, dj, dk);
+            deposit_lds<double, TN>(sD, ci - c0x, cj - c0y, ck - c0z, di, dj, dk, p.w);
+        }
+    }
+    __syncthreads();
+    const bool used = win_used[t] != 0;
+    for (int n = tid; n < WNODES; n += 256) {
+        const int a = n % WN - HALO, b = (n / WN) % WN - HALO, c = n / (WN * WN) - HALO;
+        const bool inner = a >= 0 && a < TN && b >= 0 && b < TN && c >= 0 && c < TN;
+        const double v = inner ? sD[a + TN * (b + TN * c)] : 0.0;
+        const long m = (long)t * WNODES + n;
+        if (used && v == 0.0) continue;
+        if constexpr (PREC == PIC_MIXED)
+            win32[m] = used ? (float)((double)win32[m] + v) : (float)v;
+        else
+            win64[m] = used ? win64[m] + v : v;
+    }
+    if (!used && tid == 0) win_used[t] = 1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1565,6 +1645,8 @@ static int alloc_buf(pic_species *s, PartBuf &b, long cap_bins, long cap_stage) 
     bytes += idb;
     size_t binb = (((size_t)g.ntiles * 4 + 255) & ~(size_t)255);
     bytes += binb * 5 + 256;
+    const size_t markb = ((size_t)n + 255) & ~(size_t)255;
+    bytes += markb;
     void *mem = nullptr;
     PIC_HIP(hipMalloc(&mem, bytes));
     b = PartBuf{};
@@ -1595,8 +1677,9 @@ static int alloc_buf(pic_species *s, PartBuf &b, long cap_bins, long cap_stage) 
     b.bin_own = (uint32_t *)take(binb);
     b.bin_in = (uint32_t *)take(binb);
     b.stage_count = (uint32_t *)take(256);
-    // empty bins, empty staging
-    PIC_HIP(hipMemsetAsync(b.bin_start, 0, binb * 5 + 256, s->w->stream));
+    b.mark = (uint8_t *)take(markb);
+    // empty bins, empty staging, no marks
+    PIC_HIP(hipMemsetAsync(b.bin_start, 0, binb * 5 + 256 + markb, s->w->stream));
     return PIC_OK;
 }
 
@@ -1879,7 +1962,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMalloc(&s->den_ave, sizeof(double) * g.nn));
     TRY(hipMalloc(&s->win, wbytes));
     TRY(hipMalloc(&s->win_used, g.ntiles));
-    TRY(hipMalloc(&s->far_tile, sizeof(uint32_t) * 2 * (size_t)g.ntiles));
+    TRY(hipMalloc(&s->far_tile, sizeof(uint32_t) * 3 * (size_t)g.ntiles));
     TRY(hipMalloc(&s->plan_mem, plan_bytes(g)));
     TRY(hipMalloc(&s->diag_dev, sizeof(double) * 8));
     if ((rc = ensure_push_scratch(s))) goto fail;
@@ -1887,7 +1970,7 @@ int pic_species_create(pic_world *w, double mass, double charge, double mpw0, in
     TRY(hipMemsetAsync(s->st_dev, 0, sizeof(SpStatus), w->stream));
     TRY(hipMemsetAsync(s->den, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->den_extra, 0, sizeof(double) * g.nn, w->stream));
-    TRY(hipMemsetAsync(s->far_tile, 0, sizeof(uint32_t) * 2 * (size_t)g.ntiles, w->stream));
+    TRY(hipMemsetAsync(s->far_tile, 0, sizeof(uint32_t) * 3 * (size_t)g.ntiles, w->stream));
     TRY(hipMemsetAsync(s->den_ave, 0, sizeof(double) * g.nn, w->stream));
     TRY(hipMemsetAsync(s->win_used, 0, g.ntiles, w->stream));
     TRY(hipMemsetAsync(s->plan_mem, 0, plan_bytes(g), w->stream));  // staged counts start at 0
@@ -2187,6 +2270,7 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     O.diag_part = s->diag_part;
     O.err = err_flag(s);
     O.far = s->far_tile;
+    O.far_in = s->far_tile + 2 * (size_t)g.ntiles;
     O.chunk = (uint32_t)s->push_chunk;
     O.stripe = (uint32_t)s->xcd_stripe;
     // workgroups for the tiles (the plan's map decides which run) + the staging workgroups
@@ -2196,12 +2280,18 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     timer_begin(w, PIC_T_PUSH_KERNEL, &evk);
     // the two-cell-rim push, then the wide-rim one for the tiles the plan flagged (each
     // returns at once on the other's workgroups)
+    // then the deferred far deposits (tiles the push flagged, and the staging list)
+    const unsigned far_grid = (unsigned)g.ntiles + STAGE_WGS;
     if (s->prec == PIC_MIXED) {
         k_push<PIC_MIXED, HALO><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
         k_push<PIC_MIXED, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+        k_far_arrivals<PIC_MIXED><<<far_grid, 256, 0, w->stream>>>(g, view_of(out), O.far_in, O.win32, O.win64,
+                                                                   O.win_used, s->den_extra);
     } else {
         k_push<PIC_FP64, HALO><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
         k_push<PIC_FP64, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+        k_far_arrivals<PIC_FP64><<<far_grid, 256, 0, w->stream>>>(g, view_of(out), O.far_in, O.win32, O.win64,
+                                                                  O.win_used, s->den_extra);
     }
     PIC_LAUNCH_CHECK();
     timer_end(w, PIC_T_PUSH_KERNEL, evk);

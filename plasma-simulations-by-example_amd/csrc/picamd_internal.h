@@ -106,6 +106,9 @@ struct PartBuf {
     uint32_t *bin_start = nullptr, *bin_cap = nullptr, *bin_off = nullptr;
     uint32_t *bin_own = nullptr, *bin_in = nullptr;
     uint32_t *stage_count = nullptr;  // device counter of the staging list
+    // one byte per slot: 1 for a particle whose deposit the push left to k_far_arrivals (set
+    // by the push, cleared by k_far_arrivals right after it: 0 everywhere else)
+    uint8_t *mark = nullptr;
 };
 
 // Device view of a buffer, passed by value to kernels.
@@ -115,12 +118,13 @@ struct PartView {
     uint32_t *id;
     uint32_t *bin_start, *bin_cap, *bin_off, *bin_own, *bin_in, *stage_count;
     long cap_bins, cap_stage;
+    uint8_t *mark;
 };
 
 inline PartView view_of(const PartBuf &b) {
     return PartView{b.rec, b.x, b.y, b.z,
                     b.dvx, b.dvy, b.dvz, b.dw, b.id, b.bin_start, b.bin_cap, b.bin_off,
-                    b.bin_own, b.bin_in, b.stage_count, b.cap_bins, b.cap_stage};
+                    b.bin_own, b.bin_in, b.stage_count, b.cap_bins, b.cap_stage, b.mark};
 }
 
 // the two segments of tile t: q in [0, c0+c1) lives at q < c0 ? s0+q : s1+(q-c0)
@@ -246,7 +250,8 @@ struct pic_species {
     void *win = nullptr;          // per-tile deposit windows (float or double)
     uint8_t *win_used = nullptr;
     uint32_t *far_tile = nullptr;  // per tile: far deposits of the last push, then [ntiles..) the
-                                   // plan's wide-rim flags
+                                   // plan's wide-rim flags, then [2 ntiles..) the push's
+                                   // far-arrival flags (cleared by k_far_arrivals)
     void *plan_mem = nullptr;     // scratch of the segment plan (per-tile counts, block sums)
     uint64_t *chunk_map = nullptr;  // push workgroup -> (tile, chunk), written by the plan
     long chunk_cap = 0;           // entries of chunk_map

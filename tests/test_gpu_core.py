@@ -272,7 +272,10 @@ def test_frame_velocity_change_keeps_particles():
 # The wide-rim push (a three-cell window rim for the tiles whose particles left the two-cell
 # one in the last push): forced for every tile, and chosen per tile from the far counts.  At
 # 3e7 m/s and 1e-9 s a particle moves up to 3 cells a step, so many land outside the two-cell
-# rim, and some outside the three-cell one too.
+# rim, and some outside the three-cell one too.  Those far deposits are deferred: the push
+# stores such a particle with its weight's sign bit set, and k_far_arrivals deposits it
+# through its destination tile and restores the weight, so the weights read back are checked
+# as well.
 @pytest.mark.parametrize("mode", ["all", "auto"])
 def test_push_wide_rim_fp64_matches_oracle(mode, monkeypatch):
     if mode == "all":
@@ -280,6 +283,7 @@ def test_push_wide_rim_fp64_matches_oracle(mode, monkeypatch):
     r = _push_case(P.FP64, P.REFLECT, 10, 1e-9, 3e7)
     assert np.array_equal(r["gi"], r["ri"])
     assert np.array_equal(r["gp"], r["rp"]) and np.array_equal(r["gv"], r["rv"])
+    assert np.array_equal(r["gw"], r["rw"])
     assert rel_l2(r["den"], r["den_ref"]) < 1e-13
     d = O.species_diag(r["rv"], r["rw"], O.ME)
     np.testing.assert_allclose(r["diag"], d, rtol=1e-12, atol=1e-40)
@@ -291,6 +295,7 @@ def test_push_wide_rim_mixed_matches_oracle(mode, monkeypatch):
         monkeypatch.setenv("PIC_PUSH_WIDE", "all")
     r = _push_case(P.MIXED, P.REFLECT, 10, 1e-9, 3e7)
     assert np.array_equal(r["gi"], r["ri"])
+    assert np.array_equal(r["gw"], r["rw"].astype(np.float32).astype(np.float64))  # FP32 weights
     dcell = np.abs(r["gp"] - r["rp"]) / r["g"].dh
     assert dcell.max() < 2e-4, dcell.max()
     assert rel_l2(r["den"], r["den_ref"]) < 1e-5
@@ -303,6 +308,7 @@ def test_push_wide_rim_split_tile_matches_oracle(monkeypatch):
     r = _push_case(P.FP64, P.REFLECT, 6, 1e-9, 3e6, **SPLIT)
     assert len(r["gi"]) == SPLIT["n"] and np.array_equal(r["gi"], r["ri"])
     assert np.array_equal(r["gp"], r["rp"]) and np.array_equal(r["gv"], r["rv"])
+    assert np.array_equal(r["gw"], r["rw"])
     assert rel_l2(r["den"], r["den_ref"]) < 1e-13
 
 
