@@ -563,7 +563,7 @@ def main():
                        f"times this run's particles")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": "k_push<PIC_MIXED, HALO> + k_push<PIC_MIXED, HALO_WIDE> + k_far_arrivals<PIC_MIXED>", "bytes_per_launch": bytes_launch,
+                "kernel": "k_push<PIC_MIXED, HALO> + k_far_arrivals<PIC_MIXED>", "bytes_per_launch": bytes_launch,
                 "kernel_ms": mixed["push_kernel_avg_ms"],
                 "copy_kernel_gbps": copy_gbps}
 

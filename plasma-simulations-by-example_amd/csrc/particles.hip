@@ -1690,6 +1690,19 @@ static void free_buf(PartBuf &b) {
 
 static long default_stage_cap(long cap_bins) { return std::max<long>(65536, cap_bins / 16); }
 
+// Which tiles the three-cell-rim push takes: none (the default; -1), those flagged by their
+// far deposits (PIC_PUSH_WIDE=auto; 0) or all (PIC_PUSH_WIDE=all; 1).  With the far deposits
+// deferred to k_far_arrivals, the two-cell rim alone measured faster late in the sphere run
+// (13.6 against 14.4-14.9 ms per push at steps 81-100, same early): the wide window's
+// occupancy (4 workgroups per CU against 6) costs more than the far deposits it saves.
+static int wide_policy() {
+    const char *fw = getenv("PIC_PUSH_WIDE");
+    if (!fw) return -1;
+    if (!strcmp(fw, "all")) return 1;
+    if (!strcmp(fw, "auto")) return 0;
+    return -1;
+}
+
 static PlanTmp plan_tmp(pic_species *s) {
     const Grid &g = s->w->g;
     size_t nt = (size_t)g.ntiles;
@@ -1761,9 +1774,7 @@ static int launch_plan(pic_species *s, const PartView &in, const uint32_t *hist,
     k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk, nblk, nullptr);
     k_plan_starts<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, vo);
     if (mode == 0) {  // push workgroups per tile
-        // PIC_PUSH_WIDE=all / off forces the wide-rim push for every tile / for none (tests)
-        const char *fw = getenv("PIC_PUSH_WIDE");
-        const int force = !fw ? 0 : !strcmp(fw, "all") ? 1 : !strcmp(fw, "off") ? -1 : 0;
+        const int force = wide_policy();
         k_plan_chunks<<<nblk, PLAN_THREADS, 0, w->stream>>>(g, T, s->far_tile, s->far_tile + g.ntiles, force,
                                                             (uint32_t)s->push_chunk);
         k_plan_scan<<<1, PLAN_SCAN_THREADS, 0, w->stream>>>(T.blk2, nblk, T.total);
@@ -2280,16 +2291,22 @@ int pic_species_advance(pic_species *s, int bc, double dt) {
     timer_begin(w, PIC_T_PUSH_KERNEL, &evk);
     // the two-cell-rim push, then the wide-rim one for the tiles the plan flagged (each
     // returns at once on the other's workgroups)
-    // then the deferred far deposits (tiles the push flagged, and the staging list)
+    // then the deferred far deposits (tiles the push flagged, and the staging list).  The
+    // wide-rim launch is left out when the plan gave it no tile.
     const unsigned far_grid = (unsigned)g.ntiles + STAGE_WGS;
+    const bool wide = wide_policy() >= 0;
     if (s->prec == PIC_MIXED) {
         k_push<PIC_MIXED, HALO><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
-        k_push<PIC_MIXED, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+        if (wide)
+            k_push<PIC_MIXED, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32,
+                                                                               w->ef64, O);
         k_far_arrivals<PIC_MIXED><<<far_grid, 256, 0, w->stream>>>(g, view_of(out), O.far_in, O.win32, O.win64,
                                                                    O.win_used, s->den_extra);
     } else {
         k_push<PIC_FP64, HALO><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
-        k_push<PIC_FP64, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32, w->ef64, O);
+        if (wide)
+            k_push<PIC_FP64, HALO_WIDE><<<grid, PUSH_THREADS, 0, w->stream>>>(P, view_of(in), view_of(out), w->ef32,
+                                                                              w->ef64, O);
         k_far_arrivals<PIC_FP64><<<far_grid, 256, 0, w->stream>>>(g, view_of(out), O.far_in, O.win32, O.win64,
                                                                   O.win_used, s->den_extra);
     }

@@ -270,16 +270,15 @@ def test_frame_velocity_change_keeps_particles():
 
 
 # The wide-rim push (a three-cell window rim for the tiles whose particles left the two-cell
-# one in the last push): forced for every tile, and chosen per tile from the far counts.  At
+# one in the last push; off by default): forced for every tile, and chosen per tile from the
+# far counts (PIC_PUSH_WIDE=all / auto); without it the default two-cell rim.  At
 # 3e7 m/s and 1e-9 s a particle moves up to 3 cells a step, so many land outside the two-cell
 # rim, and some outside the three-cell one too.  Those far deposits are deferred: the push
-# stores such a particle with its weight's sign bit set, and k_far_arrivals deposits it
-# through its destination tile and restores the weight, so the weights read back are checked
-# as well.
-@pytest.mark.parametrize("mode", ["all", "auto"])
+# marks such a particle's slot, and k_far_arrivals deposits it through its destination tile
+# (the density checks) and clears the mark; the weights read back are checked as well.
+@pytest.mark.parametrize("mode", ["all", "auto", "off"])
 def test_push_wide_rim_fp64_matches_oracle(mode, monkeypatch):
-    if mode == "all":
-        monkeypatch.setenv("PIC_PUSH_WIDE", "all")
+    monkeypatch.setenv("PIC_PUSH_WIDE", mode)
     r = _push_case(P.FP64, P.REFLECT, 10, 1e-9, 3e7)
     assert np.array_equal(r["gi"], r["ri"])
     assert np.array_equal(r["gp"], r["rp"]) and np.array_equal(r["gv"], r["rv"])
@@ -289,10 +288,9 @@ def test_push_wide_rim_fp64_matches_oracle(mode, monkeypatch):
     np.testing.assert_allclose(r["diag"], d, rtol=1e-12, atol=1e-40)
 
 
-@pytest.mark.parametrize("mode", ["all", "auto"])
+@pytest.mark.parametrize("mode", ["all", "auto", "off"])
 def test_push_wide_rim_mixed_matches_oracle(mode, monkeypatch):
-    if mode == "all":
-        monkeypatch.setenv("PIC_PUSH_WIDE", "all")
+    monkeypatch.setenv("PIC_PUSH_WIDE", mode)
     r = _push_case(P.MIXED, P.REFLECT, 10, 1e-9, 3e7)
     assert np.array_equal(r["gi"], r["ri"])
     assert np.array_equal(r["gw"], r["rw"].astype(np.float32).astype(np.float64))  # FP32 weights
