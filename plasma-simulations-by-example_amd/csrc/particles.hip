@@ -1071,33 +1071,56 @@ __global__ void __launch_bounds__(256)
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
     const int c0x = tx * TILE, c0y = ty * TILE, c0z = tz * TILE;
     const BinSeg bs = bin_of(out, t);
-    for (uint32_t q = tid; q < bs.c1; q += 256) {
-        const long idx = (long)bs.s1 + q;
-        if (!out.mark[idx]) continue;
-        out.mark[idx] = 0;
-        PT p;
-        load_p(out, idx, p);
-        int ci, cj, ck;
-        if constexpr (PREC == PIC_MIXED) {
-            deposit_lds<float, TN>(sD, p.ci - c0x, p.cj - c0y, p.ck - c0z, p.ox, p.oy, p.oz, p.w);
-        } else {
-            double di, dj, dk;
-            cell_of(g, p, ci, cj, ck, di, dj, dk);
-            deposit_lds<double, TN>(sD, ci - c0x, cj - c0y, ck - c0z, di, dj, dk, p.w);
+    // the marks of the inflow segment [lo, hi), 16 bytes per load (the mark array is 256-byte
+    // aligned and padded to 256 bytes; bytes of the neighbouring segments are skipped)
+    const long lo = (long)bs.s1, hi = lo + bs.c1;
+    for (long base = (lo & ~15L) + 16L * tid; base < hi; base += 16L * 256) {
+        const uint4 m4 = *reinterpret_cast<const uint4 *>(out.mark + base);
+        if ((m4.x | m4.y | m4.z | m4.w) == 0u) continue;
+        const uint32_t words[4] = {m4.x, m4.y, m4.z, m4.w};
+        for (int e = 0; e < 16; e++) {
+            if (((words[e >> 2] >> (8 * (e & 3))) & 0xffu) == 0u) continue;
+            const long idx = base + e;
+            if (idx < lo || idx >= hi) continue;
+            out.mark[idx] = 0;
+            PT p;
+            load_p(out, idx, p);
+            if constexpr (PREC == PIC_MIXED) {
+                deposit_lds<float, TN>(sD, p.ci - c0x, p.cj - c0y, p.ck - c0z, p.ox, p.oy, p.oz, p.w);
+            } else {
+                int ci, cj, ck;
+                double di, dj, dk;
+                cell_of(g, p, ci, cj, ck, di, dj, dk);
+                deposit_lds<double, TN>(sD, ci - c0x, cj - c0y, ck - c0z, di, dj, dk, p.w);
+            }
         }
     }
     __syncthreads();
     const bool used = win_used[t] != 0;
-    for (int n = tid; n < WNODES; n += 256) {
-        const int a = n % WN - HALO, b = (n / WN) % WN - HALO, c = n / (WN * WN) - HALO;
-        const bool inner = a >= 0 && a < TN && b >= 0 && b < TN && c >= 0 && c < TN;
-        const double v = inner ? sD[a + TN * (b + TN * c)] : 0.0;
-        const long m = (long)t * WNODES + n;
-        if (used && v == 0.0) continue;
-        if constexpr (PREC == PIC_MIXED)
-            win32[m] = used ? (float)((double)win32[m] + v) : (float)v;
-        else
-            win64[m] = used ? win64[m] + v : v;
+    if (used) {
+        // add the tile's (TILE+1)^3 nodes into its slot
+        for (int n = tid; n < TNODES; n += 256) {
+            const double v = sD[n];
+            if (v == 0.0) continue;
+            const int a = n % TN + HALO, b = (n / TN) % TN + HALO, c = n / (TN * TN) + HALO;
+            const long m = (long)t * WNODES + a + WN * (b + WN * c);
+            if constexpr (PREC == PIC_MIXED)
+                win32[m] = (float)((double)win32[m] + v);
+            else
+                win64[m] += v;
+        }
+    } else {
+        // the push left the slot unused: write it whole
+        for (int n = tid; n < WNODES; n += 256) {
+            const int a = n % WN - HALO, b = (n / WN) % WN - HALO, c = n / (WN * WN) - HALO;
+            const bool inner = a >= 0 && a < TN && b >= 0 && b < TN && c >= 0 && c < TN;
+            const double v = inner ? sD[a + TN * (b + TN * c)] : 0.0;
+            const long m = (long)t * WNODES + n;
+            if constexpr (PREC == PIC_MIXED)
+                win32[m] = (float)v;
+            else
+                win64[m] = v;
+        }
     }
     if (!used && tid == 0) win_used[t] = 1;
 }
