@@ -19,7 +19,7 @@ void data_array(ofstream &out, const string &name, int comps, const char *type, 
     out << "</DataArray>\n";
 }
 
-ofstream f_diag;
+ofstream diag_csv;
 
 }  // namespace
 
@@ -62,23 +62,23 @@ void Output::screenOutput(World &world, vector<Species> &species) {
 // steps.
 void Output::diagOutput(World &world, vector<Species> &species) {
     static const char *const per_species[] = {"mp_count", "real_count", "px", "py", "pz", "KE"};
-    if (!f_diag.is_open()) {
-        f_diag.open("runtime_diags.csv");
-        f_diag << "ts,time,wall_time";
+    if (!diag_csv.is_open()) {
+        diag_csv.open("runtime_diags.csv");
+        diag_csv << "ts,time,wall_time";
         for (Species &sp : species)
-            for (const char *col : per_species) f_diag << "," << col << "." << sp.name;
-        f_diag << ",PE,E_total" << endl;
+            for (const char *col : per_species) diag_csv << "," << col << "." << sp.name;
+        diag_csv << ",PE,E_total" << endl;
     }
-    f_diag << world.getTs() << "," << world.getTime() << "," << world.getWallTime();
+    diag_csv << world.getTs() << "," << world.getTime() << "," << world.getWallTime();
     double kinetic = 0;  // summed over the species in their order, then PE added
     for (Species &sp : species) {
         const double ke = sp.getKE();
         double3 p = sp.getMomentum();
-        f_diag << "," << sp.getNp();  // an integer column
-        for (double v : {sp.getRealCount(), p[0], p[1], p[2], ke}) f_diag << "," << v;
+        diag_csv << "," << sp.getNp();  // an integer column
+        for (double v : {sp.getRealCount(), p[0], p[1], p[2], ke}) diag_csv << "," << v;
         kinetic += ke;
     }
     const double pe = world.getPE();
-    f_diag << "," << pe << "," << (kinetic + pe) << "\n";
-    if (world.getTs() % 25 == 0) f_diag.flush();
+    diag_csv << "," << pe << "," << (kinetic + pe) << "\n";
+    if (world.getTs() % 25 == 0) diag_csv.flush();
 }
