@@ -1038,11 +1038,11 @@ __global__ void __launch_bounds__(PUSH_THREADS)
 // ---------------------------------------------------------------------------------------
 // Workgroup t < ntiles: when the push flagged tile t, the marked particles of its inflow
 // segment (one byte per slot, cleared here) are deposited into an LDS window of the tile's
-// (TILE+1)^3 nodes, in the push's arithmetic; the window is then
-// added into the tile's window slot (this workgroup owns it: no atomics), or written with the
-// rest of the slot zeroed when the push left the slot unused; the flag is cleared.
-// Workgroups past ntiles walk the staging list, whose marked particles (arrivals that found
-// their segment full) deposit by global atomics.
+// (TILE+1)^3 nodes, in the push's arithmetic.  The window is then added into the tile's
+// window slot (this workgroup owns it: no atomics), or written with the rest of the slot
+// zeroed when the push left the slot unused, and the flag is cleared.  Workgroups past ntiles
+// walk the staging list, whose marked particles (arrivals that found their segment full)
+// deposit by global atomics.
 template <int PREC>
 __global__ void __launch_bounds__(256)
     k_far_arrivals(Grid g, PartView out, uint32_t *far_in, float *win32, double *win64, uint8_t *win_used,
